@@ -1,0 +1,318 @@
+/*
+ * spt.h - C ABI of the MI355X (gfx950) path-tracing core.
+ *
+ * The core sits behind swpathtracer's AccelStructure / Integrator interfaces:
+ *   AccelStructure  acceleration/accel_structure.h:15-35   (build + intersect)
+ *   Integrator      integrators/integrator.h:8-55            (render + stats)
+ * Host C++ keeps scene assembly and the SAH BVH build (acceleration/sbvh.cpp);
+ * everything per ray / per path runs as hand-written HIP kernels.
+ *
+ * Conventions
+ *   - Every entry point returns spt_status (0 = SPT_OK). No C++ exception crosses
+ *     this boundary; spt_last_error() gives the message of the last failure.
+ *   - Buffers named d_* are DEVICE pointers (hipMalloc'd, or spt_malloc'd).
+ *     Buffers without the prefix are host pointers and are copied; the caller
+ *     keeps ownership.
+ *   - A context owns one HIP stream. Launches are asynchronous on that stream;
+ *     spt_sync() waits. A context is used by one host thread at a time.
+ *   - Floating-point semantics follow the reference: IEEE binary32, no FMA
+ *     contraction, correctly rounded divide and sqrt, denormals kept.
+ */
+#ifndef SPT_H
+#define SPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t spt_status;
+enum {
+    SPT_OK = 0,
+    SPT_ERR_INVALID = 1,     /* bad argument / shape */
+    SPT_ERR_DEVICE = 2,      /* HIP runtime error or missing device */
+    SPT_ERR_NOMEM = 3,       /* allocation failed */
+    SPT_ERR_STATE = 4,       /* call made in the wrong state (no scene, ...) */
+    SPT_ERR_UNSUPPORTED = 5, /* feature not available for this input */
+    SPT_ERR_OVERFLOW = 6     /* a bounded resource (replay stream, stack) ran out */
+};
+
+#define SPT_MISS 0xFFFFFFFFu
+
+/* ------------------------------------------------------------------------- */
+/* Data formats                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* A ray: Ray (core/ray.h:6-19) padded to 32 B so a lane loads it as 2 x 16 B. */
+typedef struct spt_ray32 {
+    float ox, oy, oz, pad0;
+    float dx, dy, dz, pad1;
+} spt_ray32;
+
+/* Closest hit: t and the hit primitive's id in scene insertion order
+ * (core/scene.cpp:26-30). prim_id == SPT_MISS when nothing is hit, t = +inf. */
+typedef struct spt_hit {
+    float t;
+    uint32_t prim_id;
+} spt_hit;
+
+/* One BVH node of the flattened tree: the boxes of BOTH children (an AABB pair)
+ * and their references, 64 B, loaded by one lane as 4 x 16 B.
+ * A child with cnt == 0 is the inner node nodes[ref]; with cnt > 0 it is a leaf
+ * holding triangles tris[ref .. ref+cnt). Boxes are the reference's padded child
+ * boxes (acceleration/sbvh.cpp:409-412). Node 0 holds the root's children; the
+ * root's own (unpadded) box is kept separately. */
+typedef struct spt_node64 {
+    float lmin[3];
+    float lmax[3];
+    float rmin[3];
+    float rmax[3];
+    uint32_t lref, lcnt, rref, rcnt;
+} spt_node64;
+
+/* One triangle in leaf (depth-first, left-first) order: v0 and the edges
+ * e1 = v1 - v0, e2 = v2 - v0 exactly as primitives/triangle.cpp:22-23 computes
+ * them, plus the primitive id. 48 B = 3 x 16 B. */
+typedef struct spt_tri48 {
+    float v0[3];
+    float e1[3];
+    float e2[3];
+    uint32_t prim_id;
+    uint32_t pad[2];
+} spt_tri48;
+
+/* BSDF layer kinds (the reference materials/ directory). GGX_CONDUCTOR has no reference
+ * counterpart: Walter et al. 2007 GGX with an exact complex-IOR conductor
+ * Fresnel term. */
+enum {
+    SPT_BSDF_NONE = 0,
+    SPT_BSDF_LAMBERTIAN = 1,        /* materials/lambertian.cpp            */
+    SPT_BSDF_COOK_TORRANCE = 2,     /* materials/cook_torrance.cpp (Beckmann) */
+    SPT_BSDF_SMOOTH_CONDUCTOR = 3,  /* materials/smooth_conductor.cpp      */
+    SPT_BSDF_SMOOTH_DIELECTRIC = 4, /* materials/smooth_dielectric.cpp     */
+    SPT_BSDF_GGX_CONDUCTOR = 5
+};
+/* bsdf_sampler = "importance" | "uniform" (lua_core/bsdfs.lua:9). */
+enum { SPT_SAMPLER_IMPORTANCE = 0, SPT_SAMPLER_UNIFORM = 1 };
+
+typedef struct spt_layer {
+    uint32_t type;     /* SPT_BSDF_* */
+    uint32_t sampler;  /* SPT_SAMPLER_* */
+    float kd[3];       /* Lambertian reflectance */
+    float r0[3];       /* Schlick reflectance at normal incidence */
+    float roughness;   /* Beckmann m (Cook-Torrance) or GGX alpha */
+    float ior;         /* smooth dielectric IOR (outside medium: 1.0) */
+    float eta[3];      /* GGX conductor: complex IOR, real part per RGB */
+    float k[3];        /* GGX conductor: complex IOR, imaginary part per RGB */
+} spt_layer;           /* 64 B */
+
+#define SPT_MAX_LAYERS 4
+
+/* Material = layered BSDF + emission (materials/material.h, layered_bsdf.h).
+ * layers[0] is the top layer. */
+typedef struct spt_material {
+    uint32_t n_layers;
+    float emission[3];
+    spt_layer layers[SPT_MAX_LAYERS];
+} spt_material;
+
+enum { SPT_CAMERA_PINHOLE = 0, SPT_CAMERA_ORTHOGRAPHIC = 1 };
+
+/* cameras/pinhole_camera.cpp, cameras/orthographic_camera.cpp, with the
+ * image size of core/image_buffer.h. */
+typedef struct spt_camera {
+    uint32_t type;
+    uint32_t width, height;
+    uint32_t pad;
+    float position[3];
+    float look_at[3];
+    float up[3];
+    float fov_deg;                  /* pinhole: vertical field of view */
+    float min_x, max_x, min_y, max_y; /* orthographic window */
+} spt_camera;
+
+/* pixel_samplers/ directory of the reference */
+enum { SPT_PIXEL_REGULAR = 0, SPT_PIXEL_JITTERED = 1, SPT_PIXEL_UNIFORM = 2 };
+
+/* Random numbers. The reference draws from per-thread mt19937 streams with
+ * non-deterministic seeds (core/prng.h:12-51); here every draw is a pure
+ * function of (seed, path, k), k = index of the draw within the path:
+ *   mix(z)  = SplitMix64 finaliser (Steele, Lea, Flood 2014)
+ *   key     = mix(seed ^ (path * 0x9E3779B97F4A7C15))
+ *   u(k)    = (mix(key + (k + 1) * 0x9E3779B97F4A7C15) >> 40) * 2^-24   in [0, 1)
+ * path = (pixel_y * width + pixel_x) * spp + sample. In replay mode the draws
+ * instead come from a host stream u_stream[path_local * stride + k]. */
+enum { SPT_RNG_COUNTER = 0, SPT_RNG_STREAM = 1 };
+
+/* Flags of spt_render_params.flags */
+enum {
+    SPT_RENDER_FILM32 = 1u << 0,    /* accumulate FP32 film into d_rgb32 */
+    SPT_RENDER_FILM64 = 1u << 1,    /* accumulate FP64 film into d_rgb64 (oracle film) */
+    SPT_RENDER_PER_PATH = 1u << 2,  /* write each path's radiance into d_path_rgb */
+    SPT_RENDER_NO_SORT = 1u << 3,   /* skip the material sort between stages */
+    SPT_RENDER_GUIDED = 1u << 4,    /* sample with the learned guide (spt_guide_*) */
+    SPT_RENDER_FETCH_STATS = 1u << 5, /* count node / triangle record fetches of the
+                                         extension rays (slower traversal variant) */
+    SPT_RENDER_TIME_KERNELS = 1u << 6 /* time every extend launch with HIP events */
+};
+
+/* Integrator settings. The reference has two modes (integrators/pathtracer.cpp:18,43):
+ *   MAX_DEPTH:        max_depth = path_length, rr_depth = -1
+ *   RUSSIAN_ROULETTE: max_depth = -1, rr_depth = path_length (1000-bounce cap kept)
+ * Setting both gives RR from rr_depth with a hard cap at max_depth. */
+typedef struct spt_render_params {
+    uint32_t pixel_sampler; /* SPT_PIXEL_* */
+    uint32_t spp;           /* regular/jittered round up to ceil(sqrt(spp))^2 */
+    int32_t max_depth;      /* depth >= max_depth returns 0 before tracing; < 0: none */
+    int32_t rr_depth;       /* Russian roulette when depth >= rr_depth; < 0: none */
+    uint64_t seed;
+    uint32_t spp_begin;     /* render samples [spp_begin, spp_begin + spp_count) */
+    uint32_t spp_count;     /* 0 = all */
+    uint32_t viewport[4];   /* top, left, width, height; width == 0: whole image */
+    float background[3];    /* returned by a primary miss (core/scene.h:149) */
+    uint32_t flags;         /* SPT_RENDER_* */
+    uint32_t rng;           /* SPT_RNG_* */
+    uint32_t stream_stride; /* SPT_RNG_STREAM: draws per path */
+    const float* d_stream;  /* SPT_RNG_STREAM: device buffer, paths x stride */
+} spt_render_params;
+
+/* Statistics of the built tree: StatsSBVH (acceleration/sbvh.h:19-40) plus the
+ * flattened-layout figures. */
+typedef struct spt_bvh_info {
+    uint64_t n_prims;
+    uint64_t n_nodes;       /* reference nodes (inner + leaves) */
+    uint64_t n_inner;
+    uint64_t n_leaves;
+    uint64_t tree_height;   /* levels - 1, as collectTreeStatistics counts */
+    uint64_t n_references;
+    uint64_t min_leaf, max_leaf;
+    uint64_t n_pair_nodes;  /* spt_node64 records */
+    uint64_t max_depth;     /* longest root-to-leaf path in inner nodes */
+    float root_min[3];
+    float root_max[3];
+    double build_ms;
+} spt_bvh_info;
+
+typedef struct spt_bvh spt_bvh;
+typedef struct spt_ctx spt_ctx;
+
+/* ------------------------------------------------------------------------- */
+/* Host: BVH build (replaces SBVH::build, acceleration/sbvh.cpp:12-37)        */
+/* ------------------------------------------------------------------------- */
+
+/* Builds the reference's object-split full-sweep SAH BVH over n_tris triangles
+ * (verts: n_tris x 9 floats, v0 v1 v2) with the reference's sort sequence, SAH
+ * constants (1.2 / 1.0), leaf rule and +-1e-3 child padding, so the tree is the
+ * one the reference would build for the same primitive order. n_threads > 1
+ * builds independent subtrees concurrently (same tree). */
+spt_status spt_bvh_build(const float* verts, uint32_t n_tris, uint32_t n_threads, spt_bvh** out);
+spt_status spt_bvh_get_info(const spt_bvh* bvh, spt_bvh_info* info);
+/* Copies out the flattened tree: info.n_pair_nodes nodes, n_prims triangles,
+ * and leaf_prim[i] = prim id of the i-th triangle in leaf order. Any pointer may
+ * be NULL. root_box = {min xyz, max xyz}. */
+spt_status spt_bvh_export(const spt_bvh* bvh, spt_node64* nodes, spt_tri48* tris, uint32_t* leaf_prim,
+                          float* root_box);
+/* Writes the reference's tree dump text format (acceleration/sbvh.cpp:577-620). */
+spt_status spt_bvh_dump(const spt_bvh* bvh, const char* path);
+void spt_bvh_destroy(spt_bvh* bvh);
+
+/* ------------------------------------------------------------------------- */
+/* Device context                                                            */
+/* ------------------------------------------------------------------------- */
+
+spt_status spt_create(int device, spt_ctx** out);
+void spt_destroy(spt_ctx* ctx);
+const char* spt_last_error(const spt_ctx* ctx);
+spt_status spt_sync(spt_ctx* ctx);
+/* The context's hipStream_t, for callers that time or order work around it. */
+void* spt_stream(spt_ctx* ctx);
+
+spt_status spt_malloc(spt_ctx* ctx, size_t bytes, void** d_ptr);
+spt_status spt_free(spt_ctx* ctx, void* d_ptr);
+spt_status spt_memcpy_h2d(spt_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+spt_status spt_memcpy_d2h(spt_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+
+/* Uploads a built tree (copied). normals: n_prims x 9 floats (n0 n1 n2, smooth
+ * triangle, primitives/smooth_triangle.h:11-13) or NULL for flat triangles
+ * (normal = normalize(cross(e1, e2)), primitives/triangle.cpp:5-6); a row of
+ * nine zeros marks one flat triangle among smooth ones.
+ * mat_ids: n_prims material indices or NULL (all 0). */
+spt_status spt_set_scene(spt_ctx* ctx, const spt_bvh* bvh, const float* verts, const float* normals,
+                         const uint32_t* mat_ids);
+/* Same, from the flattened arrays (as produced by spt_bvh_export). shading is
+ * n_tris x 12 floats in leaf order: n0 n1 n2 (9), material id (as u32), smooth
+ * flag (as u32), pad. */
+spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_nodes, const spt_tri48* tris,
+                            uint32_t n_tris, const float* root_box, const float* shading);
+spt_status spt_set_materials(spt_ctx* ctx, const spt_material* mats, uint32_t n_mats);
+
+/* ------------------------------------------------------------------------- */
+/* Closest-hit ray casting (replaces AccelStructure::intersect, batched)      */
+/* ------------------------------------------------------------------------- */
+
+enum {
+    SPT_TRACE_CLOSEST = 0,   /* pruned near-first traversal; same closest hit */
+    SPT_TRACE_COUNT = 1,     /* visits the reference's exact node set and also
+                                returns its counters (sbvh.cpp:478-510) */
+    SPT_TRACE_UNPRUNED = 2,  /* like COUNT without writing counters */
+    SPT_TRACE_FETCH = 3      /* CLOSEST, and d_tests/d_isects receive the number of
+                                64-B node and 48-B triangle records each ray fetched */
+};
+
+/* d_rays: n rays; d_hits: n hits. In SPT_TRACE_COUNT mode d_tests/d_isects
+ * (n each, may be NULL) receive num_intersection_tests / num_intersections. */
+spt_status spt_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits, uint32_t mode,
+                     uint32_t* d_tests, uint32_t* d_isects);
+
+/* Primary rays: d_rays[y * w + x] = camera ray through image point
+ * (x + sx, y + sy) with (sx, sy) = (0.5, 0.5), the single sample of
+ * RegularPixelSampler(1) that the ray casters use (integrators/raycaster.h:45). */
+spt_status spt_camera_rays(spt_ctx* ctx, const spt_camera* cam, spt_ray32* d_rays);
+
+/* Ray casters (integrators/raycaster.h:61-99): one ray per pixel centre.
+ * kind 0 normal (d_out: w*h*3 floats, 0 on miss), 1 depth (w*h floats, t or -1),
+ * 2 intersection-test count (w*h uint32). Pixel (x, y) is at index y*w + x. */
+enum { SPT_RAYCAST_NORMAL = 0, SPT_RAYCAST_DEPTH = 1, SPT_RAYCAST_COUNT = 2 };
+spt_status spt_raycast(spt_ctx* ctx, const spt_camera* cam, uint32_t kind, void* d_out);
+
+/* ------------------------------------------------------------------------- */
+/* Path tracing (replaces PathTracer::render, integrators/pathtracer.cpp)     */
+/* ------------------------------------------------------------------------- */
+
+/* Renders samples [spp_begin, spp_begin+spp_count) of every viewport pixel.
+ * d_rgb32 (w*h*3 floats) / d_rgb64 (w*h*3 doubles) accumulate radiance SUMS
+ * (divide by the sample count to get the reference's pixel value). d_path_rgb
+ * receives 3 floats per path, path index = (pixel_in_viewport * spp_count +
+ * sample - spp_begin). Unused outputs may be NULL. */
+spt_status spt_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* params, float* d_rgb32,
+                      double* d_rgb64, float* d_path_rgb);
+
+/* Counters of the last spt_render call: paths, extension rays traced, bounce
+ * iterations, host wall ms; with SPT_RENDER_TIME_KERNELS the summed device time
+ * and count of the extend (closest-hit) launches; with SPT_RENDER_FETCH_STATS the
+ * node / triangle records those rays fetched. */
+typedef struct spt_render_stats {
+    uint64_t paths;
+    uint64_t rays;
+    uint64_t bounces;
+    uint64_t stream_overflows;
+    double ms;
+    double extend_ms;
+    uint64_t extend_launches;
+    uint64_t node_fetches;
+    uint64_t tri_fetches;
+} spt_render_stats;
+spt_status spt_render_get_stats(spt_ctx* ctx, spt_render_stats* out);
+
+/* HIP-event timing on the context stream: spt_event_record(ctx, slot) with
+ * slot in [0, 8); spt_event_elapsed() waits for slot b and returns ms a -> b. */
+spt_status spt_event_record(spt_ctx* ctx, int slot);
+spt_status spt_event_elapsed(spt_ctx* ctx, int a, int b, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPT_H */

@@ -1,0 +1,1290 @@
+// oracle.cpp - CPU restatement of swpathtracer's hot path (reference files cited
+// per function). TEST INFRASTRUCTURE ONLY: the checker for the HIP core and the
+// timed CPU baseline; the product never links it.
+//
+// Built with -O2 -ffp-contract=off and no -march, like the reference's Release
+// build on x86-64 (no FMA), so every float expression rounds as written.
+//
+// Vector arithmetic restates GLM 0.9.9's scalar formulas (the reference's
+// dependency, not vendored there): dot = (x*x' + y*y') + z*z';
+// cross(a, b) = (a.y*b.z - b.y*a.z, a.z*b.x - b.z*a.x, a.x*b.y - b.x*a.y);
+// normalize(v) = v * (1 / sqrt(dot(v, v))); min(a, b) = (b < a) ? b : a;
+// max(a, b) = (a < b) ? b : a; mat3 * v = col0*v.x + col1*v.y + col2*v.z.
+// Parity unpinned at the ulp level for: GLM's operation order (restated from
+// its published formulas), libm rounding of tan/exp/log/atan/sin/cos, and the
+// float-vs-double overload chosen by unqualified calls (noted where it occurs).
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <memory>
+#include <queue>
+#include <stdexcept>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+static const float PI_f = 3.141592653589793238462643383279502884f;  // core/core.h:4
+static const float kInf = std::numeric_limits<float>::infinity();
+
+// ---------------------------------------------------------------------------
+// vec3 with GLM's scalar formulas
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+    V3() : x(0.0f), y(0.0f), z(0.0f) {}
+    V3(float a, float b, float c) : x(a), y(b), z(c) {}
+    explicit V3(float s) : x(s), y(s), z(s) {}
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+static inline V3 operator+(V3 a, V3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 operator-(V3 a, V3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V3 operator-(V3 a) { return V3(-a.x, -a.y, -a.z); }
+static inline V3 operator*(V3 a, V3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline V3 operator*(V3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+static inline V3 operator*(float s, V3 a) { return V3(s * a.x, s * a.y, s * a.z); }
+static inline V3 operator/(V3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
+static inline V3 operator/(V3 a, V3 b) { return V3(a.x / b.x, a.y / b.y, a.z / b.z); }
+static inline V3 operator-(V3 a, float s) { return V3(a.x - s, a.y - s, a.z - s); }
+static inline V3 operator+(V3 a, float s) { return V3(a.x + s, a.y + s, a.z + s); }
+static inline V3 operator-(float s, V3 a) { return V3(s - a.x, s - a.y, s - a.z); }
+static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 cross(V3 a, V3 b) {
+    return V3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+static inline V3 normalize(V3 v) { return v * (1.0f / std::sqrt(dot(v, v))); }
+static inline float gmin(float a, float b) { return (b < a) ? b : a; }
+static inline float gmax(float a, float b) { return (a < b) ? b : a; }
+static inline V3 vmin(V3 a, V3 b) { return V3(gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)); }
+static inline V3 vmax(V3 a, V3 b) { return V3(gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)); }
+static inline float smax(float a, float b) { return (a < b) ? b : a; }  // std::max
+static inline float smin(float a, float b) { return (b < a) ? b : a; }  // std::min
+
+// 3x3 basis stored by columns (glm::mat3 m[c] is column c)
+struct M3 {
+    V3 c0, c1, c2;
+};
+static inline V3 mul(const M3& m, V3 v) {
+    return V3(m.c0.x * v.x + m.c1.x * v.y + m.c2.x * v.z, m.c0.y * v.x + m.c1.y * v.y + m.c2.y * v.z,
+              m.c0.z * v.x + m.c1.z * v.y + m.c2.z * v.z);
+}
+static inline M3 transpose(const M3& m) {
+    return M3{V3(m.c0.x, m.c1.x, m.c2.x), V3(m.c0.y, m.c1.y, m.c2.y), V3(m.c0.z, m.c1.z, m.c2.z)};
+}
+
+struct Ray {
+    V3 o, d;
+};
+
+// core/intersection_record.h:6-12 (+ the accepted primitive's id, which the
+// reference declares but never writes)
+struct Rec {
+    float t = kInf;
+    V3 pos;
+    V3 normal;
+    uint32_t material_id = 0;
+    uint32_t prim_id = SPT_MISS;
+};
+
+// ---------------------------------------------------------------------------
+// ONB  (core/onb.cpp:3-35)
+// ---------------------------------------------------------------------------
+struct ONB {
+    M3 l2w, w2l;
+    static ONB fromUp(V3 v) {  // onb.cpp:3-16
+        V3 w;
+        if (std::fabs(v.x) > std::fabs(v.y))
+            w = normalize(V3(v.z, 0.0f, -v.x));
+        else
+            w = normalize(V3(0.0f, -v.z, v.y));
+        V3 u = cross(v, w);
+        ONB o;
+        o.l2w = M3{u, v, w};
+        o.w2l = transpose(o.l2w);
+        return o;
+    }
+    static ONB fromUW(V3 u, V3 w) {  // onb.cpp:18-27
+        V3 v = cross(w, u);
+        ONB o;
+        o.l2w = M3{u, v, w};
+        o.w2l = transpose(o.l2w);
+        return o;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// AABB  (acceleration/aabb.{h,cpp})
+// ---------------------------------------------------------------------------
+struct AABB {
+    V3 mn, mx, c;
+    AABB() {}
+    AABB(V3 a, V3 b) : mn(a), mx(b), c(0.5f * (a + b)) {}
+    void setMin(V3 v) { mn = v; c = 0.5f * (mn + mx); }
+    void setMax(V3 v) { mx = v; c = 0.5f * (mn + mx); }
+    float area() const {  // aabb.cpp:11-16
+        float l = (mx.x - mn.x);
+        float m = (mx.y - mn.y);
+        float n = (mx.z - mn.z);
+        return 2.0f * (l * (m + n) + m * n);
+    }
+    AABB operator+(const AABB& r) const { return AABB(vmin(mn, r.mn), vmax(mx, r.mx)); }  // aabb.h:30-32
+    // Williams, Barrus, Morley, Shirley slab test, aabb.cpp:31-99. No t-range check.
+    bool intersect(const Ray& ray) const {
+        float tmin, tmax, tymin, tymax, tzmin, tzmax, divx, divy, divz;
+        divx = 1.0f / ray.d.x;
+        if (divx >= 0.0f) {
+            tmin = (mn.x - ray.o.x) * divx;
+            tmax = (mx.x - ray.o.x) * divx;
+        } else {
+            tmin = (mx.x - ray.o.x) * divx;
+            tmax = (mn.x - ray.o.x) * divx;
+        }
+        divy = 1.0f / ray.d.y;
+        if (divy >= 0.0f) {
+            tymin = (mn.y - ray.o.y) * divy;
+            tymax = (mx.y - ray.o.y) * divy;
+        } else {
+            tymin = (mx.y - ray.o.y) * divy;
+            tymax = (mn.y - ray.o.y) * divy;
+        }
+        if ((tmin > tymax) || (tymin > tmax)) return false;
+        if (tymin > tmin) tmin = tymin;
+        if (tymax < tmax) tmax = tymax;
+        divz = 1.0f / ray.d.z;
+        if (divz >= 0.0f) {
+            tzmin = (mn.z - ray.o.z) * divz;
+            tzmax = (mx.z - ray.o.z) * divz;
+        } else {
+            tzmin = (mx.z - ray.o.z) * divz;
+            tzmax = (mn.z - ray.o.z) * divz;
+        }
+        if ((tmin > tzmax) || (tzmin > tmax)) return false;
+        return true;
+    }
+    bool inside(V3 p) const {  // aabb.cpp:101-105
+        const float e = 1e-3f;
+        return (p.x >= (mn.x - e)) && (p.x <= (mx.x + e)) && (p.y >= (mn.y - e)) && (p.y <= (mx.y + e)) &&
+               (p.z >= (mn.z - e)) && (p.z <= (mx.z + e));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Primitives  (primitives/*.{h,cpp}): heap objects behind virtual calls, with
+// the triangle's vertices in their own heap vector, as the reference has them.
+// ---------------------------------------------------------------------------
+struct Primitive {
+    uint32_t material_id;
+    explicit Primitive(uint32_t m) : material_id(m) {}
+    virtual ~Primitive() {}
+    virtual bool intersect(const Ray& ray, Rec& rec) const = 0;
+    virtual AABB getAABB() const = 0;
+    virtual V3 getCentroid() const = 0;
+};
+
+struct Triangle : Primitive {
+    std::vector<V3> v_;
+    V3 normal_;
+    Triangle(V3 a, V3 b, V3 c, uint32_t m)  // triangle.cpp:5-6
+        : Primitive(m), v_{a, b, c}, normal_(normalize(cross(v_[1] - v_[0], v_[2] - v_[0]))) {}
+    virtual V3 getNormal(float, float) const { return normal_; }
+    // Moller-Trumbore, non-culling branch, triangle.cpp:12-74
+    bool intersect(const Ray& ray, Rec& rec) const override {
+        V3 edge1 = v_[1] - v_[0];
+        V3 edge2 = v_[2] - v_[0];
+        V3 pvec = cross(ray.d, edge2);
+        float det = dot(edge1, pvec);
+        if ((det > -1e-6f) && (det < 1e-6f)) return false;
+        float inv_det = 1.0f / det;
+        V3 tvec = ray.o - v_[0];
+        float u = dot(tvec, pvec) * inv_det;
+        if ((u < 0.0f) || (u > 1.0f)) return false;
+        V3 qvec = cross(tvec, edge1);
+        float v = dot(ray.d, qvec) * inv_det;
+        if ((v < 0.0f) || ((u + v) > 1.0f)) return false;
+        float t = dot(edge2, qvec) * inv_det;
+        rec.t = t;
+        rec.pos = ray.o + ray.d * t;
+        rec.normal = getNormal(u, v);
+        rec.material_id = material_id;
+        return true;
+    }
+    AABB getAABB() const override {  // triangle.cpp:76-83
+        AABB b;
+        b.setMin(vmin(vmin(v_[0], v_[1]), v_[2]));
+        b.setMax(vmax(vmax(v_[0], v_[1]), v_[2]));
+        return b;
+    }
+    V3 getCentroid() const override { return (v_[0] + v_[1] + v_[2]) * (1.0f / 3.0f); }  // triangle.cpp:85-87
+};
+
+struct SmoothTriangle : Triangle {  // primitives/smooth_triangle.h:6-19
+    V3 n0_, n1_, n2_;
+    SmoothTriangle(V3 a, V3 b, V3 c, V3 n0, V3 n1, V3 n2, uint32_t m) : Triangle(a, b, c, m), n0_(n0), n1_(n1), n2_(n2) {}
+    V3 getNormal(float u, float v) const override { return normalize(n0_ * (1.0f - u - v) + n1_ * u + n2_ * v); }
+};
+
+struct Sphere : Primitive {  // primitives/sphere.cpp
+    V3 center_;
+    float radius_;
+    Sphere(V3 c, float r, uint32_t m) : Primitive(m), center_(c), radius_(r) {}
+    bool intersect(const Ray& ray, Rec& rec) const override {  // sphere.cpp:6-38 (Hultquist)
+        V3 eo = center_ - ray.o;
+        float v = dot(eo, ray.d);
+        float disc = (radius_ * radius_) - (dot(eo, eo) - (v * v));
+        if (disc < 0.0f) return false;
+        float d = sqrtf(disc);
+        float t1 = v - d, t2 = v + d;
+        rec.t = (t1 > 1e-5f) ? t1 : t2;
+        rec.pos = ray.o + rec.t * ray.d;
+        rec.normal = normalize(rec.pos - center_);
+        rec.material_id = material_id;
+        return true;
+    }
+    AABB getAABB() const override {
+        AABB b;
+        b.setMin(center_ - radius_);
+        b.setMax(center_ + radius_);
+        return b;
+    }
+    V3 getCentroid() const override { return center_; }
+};
+
+// ---------------------------------------------------------------------------
+// SBVH (object splits only)  acceleration/sbvh.cpp, sbvh_node.h, primitive_ref.h
+// ---------------------------------------------------------------------------
+struct PrimRef {  // primitive_ref.h:11-17
+    long id;
+    AABB aabb;
+    V3 centroid;
+};
+
+struct BNode {  // sbvh_node.h:10-53
+    std::unique_ptr<std::vector<PrimRef>> refs;
+    AABB aabb;
+    std::unique_ptr<BNode> left, right;
+    void sortRefs(int axis) {  // sbvh_node.h:26-38 (std::sort, unstable, as the reference)
+        switch (axis) {
+            case 0:
+                std::sort(refs->begin(), refs->end(),
+                          [](const PrimRef& a, const PrimRef& b) { return a.centroid.x < b.centroid.x; });
+                break;
+            case 1:
+                std::sort(refs->begin(), refs->end(),
+                          [](const PrimRef& a, const PrimRef& b) { return a.centroid.y < b.centroid.y; });
+                break;
+            case 2:
+                std::sort(refs->begin(), refs->end(),
+                          [](const PrimRef& a, const PrimRef& b) { return a.centroid.z < b.centroid.z; });
+                break;
+        }
+    }
+};
+
+struct BVHStats {
+    uint64_t nodes = 0, inner = 0, leaves = 0, height = 0, refs = 0;
+    uint64_t min_leaf = std::numeric_limits<uint64_t>::max(), max_leaf = 0, unreferenced = 0;
+};
+
+class SBVH {
+   public:
+    const std::vector<std::unique_ptr<Primitive>>* prims = nullptr;
+    std::unique_ptr<BNode> root;
+    BVHStats stats;
+    const float cost_tri = 1.0f, cost_aabb = 1.2f;  // sbvh.cpp:6-7
+
+    // sbvh.cpp:47-56
+    float SAH(std::size_t s1_size, float s1_area, std::size_t s2_size, float s2_area, float area) const {
+        if ((s1_area == kInf) && (s1_size == 0)) return kInf;
+        if ((s2_area == kInf) && (s2_size == 0)) return kInf;
+        return 2.0f * cost_aabb + ((s1_area / area) * s1_size * cost_tri) + ((s2_area / area) * s2_size * cost_tri);
+    }
+
+    void build() {  // sbvh.cpp:12-37
+        if (prims->empty()) return;
+        root.reset(new BNode());
+        root->refs.reset(new std::vector<PrimRef>());
+        for (std::size_t i = 0; i < prims->size(); ++i) {
+            root->refs->push_back(PrimRef{(long)i, (*prims)[i]->getAABB(), (*prims)[i]->getCentroid()});
+            if (!i)
+                root->aabb = (*root->refs)[i].aabb;
+            else
+                root->aabb = root->aabb + (*root->refs)[i].aabb;
+        }
+        splitNode(root);
+        collectStats();
+    }
+
+    void splitNode(std::unique_ptr<BNode>& node) {  // sbvh.cpp:58-121, 306-421 (object split branch)
+        const std::size_t N = node->refs->size();
+        if (N == 0) throw std::runtime_error("SBVH: empty node");
+        float best_cost = cost_tri * N;
+        int best_axis = -1;
+        long best_event = -1;
+        for (int axis = 0; axis < 3; ++axis) {
+            node->sortRefs(axis);
+            std::vector<AABB> left_aabb(N - 1);
+            std::vector<float> left_area(N - 1);
+            for (std::size_t i = 0; i < N - 1; ++i) {
+                if (i == 0) {
+                    left_aabb[0] = (*node->refs)[0].aabb;
+                    left_area[0] = left_aabb[0].area();
+                } else {
+                    left_aabb[i] = left_aabb[i - 1] + (*node->refs)[i].aabb;
+                    left_area[i] = left_aabb[i].area();
+                }
+            }
+            std::vector<AABB> right_aabb(N - 1);
+            std::vector<float> right_area(N - 1);
+            for (long i = (long)N - 2; i >= 0; --i) {
+                if (i == (long)(N - 2)) {
+                    right_aabb[i] = (*node->refs)[i + 1].aabb;
+                    right_area[i] = right_aabb[i].area();
+                } else {
+                    right_aabb[i] = right_aabb[i + 1] + (*node->refs)[i + 1].aabb;
+                    right_area[i] = right_aabb[i].area();
+                }
+                float this_cost = SAH(i + 1, left_area[i], N - 1 - i, right_area[i], node->aabb.area());
+                if (this_cost < best_cost) {
+                    best_cost = this_cost;
+                    best_event = i;
+                    best_axis = axis;
+                }
+            }
+        }
+        if (best_axis == -1) return;  // leaf: keeps its reference list
+        node->sortRefs(best_axis);     // sbvh.cpp:310
+        node->left.reset(new BNode());
+        node->left->refs.reset(
+            new std::vector<PrimRef>(node->refs->begin(), node->refs->begin() + best_event + 1));
+        for (std::size_t i = 0; i < node->left->refs->size(); ++i)
+            node->left->aabb = i ? node->left->aabb + (*node->left->refs)[i].aabb : (*node->left->refs)[i].aabb;
+        node->right.reset(new BNode());
+        node->right->refs.reset(new std::vector<PrimRef>(node->refs->begin() + best_event + 1, node->refs->end()));
+        for (std::size_t i = 0; i < node->right->refs->size(); ++i)
+            node->right->aabb =
+                i ? node->right->aabb + (*node->right->refs)[i].aabb : (*node->right->refs)[i].aabb;
+        // child padding, sbvh.cpp:409-412
+        node->left->aabb.setMin(node->left->aabb.mn - 1e-3f);
+        node->left->aabb.setMax(node->left->aabb.mx + 1e-3f);
+        node->right->aabb.setMin(node->right->aabb.mn - 1e-3f);
+        node->right->aabb.setMax(node->right->aabb.mx + 1e-3f);
+        node->refs.reset();
+        splitNode(node->left);
+        splitNode(node->right);
+    }
+
+    static bool isLeaf(const BNode* n) { return !n->left && !n->right; }
+
+    // sbvh.cpp:478-510 (active branch): left then right, no pruning.
+    bool traverse(const BNode* node, const Ray& ray, Rec& rec, std::size_t& tests, std::size_t& hits) const {
+        bool hit = false;
+        if (node) {
+            ++tests;
+            if (node->aabb.intersect(ray)) {
+                ++hits;
+                if (isLeaf(node)) {
+                    Rec tmp;
+                    for (std::size_t i = 0; i < node->refs->size(); ++i) {
+                        ++tests;
+                        const long id = (*node->refs)[i].id;
+                        if ((*prims)[id]->intersect(ray, tmp)) {
+                            ++hits;
+                            if ((tmp.t > 0.0f) && (tmp.t < rec.t) && node->aabb.inside(tmp.pos)) {
+                                rec = tmp;
+                                rec.prim_id = (uint32_t)id;
+                                hit = true;
+                            }
+                        }
+                    }
+                } else {
+                    if (traverse(node->left.get(), ray, rec, tests, hits)) hit = true;
+                    if (traverse(node->right.get(), ray, rec, tests, hits)) hit = true;
+                }
+            }
+        }
+        return hit;
+    }
+
+    bool intersect(const Ray& ray, Rec& rec, std::size_t& tests, std::size_t& hits) const {  // sbvh.cpp:39-45
+        rec.t = kInf;
+        tests = 0;
+        hits = 0;
+        return traverse(root.get(), ray, rec, tests, hits);
+    }
+
+    void collectStats() {  // sbvh.cpp:517-575
+        std::queue<BNode*> q;
+        std::vector<unsigned> seen(prims->size(), 0);
+        if (root) {
+            q.push(root.get());
+            q.push(nullptr);
+            while (!q.empty()) {
+                BNode* n = q.front();
+                q.pop();
+                if (n) {
+                    if (isLeaf(n)) {
+                        ++stats.leaves;
+                        stats.refs += n->refs->size();
+                        stats.min_leaf = std::min<uint64_t>(stats.min_leaf, n->refs->size());
+                        stats.max_leaf = std::max<uint64_t>(stats.max_leaf, n->refs->size());
+                        for (auto& r : *n->refs) ++seen[r.id];
+                    } else {
+                        ++stats.inner;
+                    }
+                    ++stats.nodes;
+                    if (n->left) q.push(n->left.get());
+                    if (n->right) q.push(n->right.get());
+                } else if (!q.empty()) {
+                    q.push(nullptr);
+                    ++stats.height;
+                }
+            }
+        }
+        for (auto c : seen)
+            if (!c) ++stats.unreferenced;
+    }
+
+    bool dump(const char* path) const {  // sbvh.cpp:577-620
+        std::ofstream f(path);
+        if (!f) return false;
+        std::queue<BNode*> q;
+        if (root) {
+            q.push(root.get());
+            q.push(nullptr);
+            f << "\n";
+            while (!q.empty()) {
+                BNode* n = q.front();
+                q.pop();
+                if (n) {
+                    f << (isLeaf(n) ? "l" : "i");
+                    f << " " << n->aabb.mn.x << " " << n->aabb.mn.y << " " << n->aabb.mn.z;
+                    f << " " << n->aabb.mx.x << " " << n->aabb.mx.y << " " << n->aabb.mx.z;
+                    f << "\n";
+                    if (n->left) q.push(n->left.get());
+                    if (n->right) q.push(n->right.get());
+                } else if (!q.empty()) {
+                    q.push(nullptr);
+                    f << "\n";
+                }
+            }
+        }
+        return true;
+    }
+
+    void leafOrder(const BNode* n, std::vector<uint32_t>& out) const {
+        if (!n) return;
+        if (isLeaf(n)) {
+            for (auto& r : *n->refs) out.push_back((uint32_t)r.id);
+            return;
+        }
+        leafOrder(n->left.get(), out);
+        leafOrder(n->right.get(), out);
+    }
+};
+
+// acceleration/no_accel_structure.cpp:9-30
+static bool bruteIntersect(const std::vector<std::unique_ptr<Primitive>>& prims, const Ray& ray, Rec& rec,
+                           std::size_t& tests, std::size_t& hits) {
+    Rec tmp;
+    rec.t = kInf;
+    bool result = false;
+    tests = 0;
+    hits = 0;
+    for (std::size_t i = 0; i < prims.size(); ++i) {
+        ++tests;
+        if (prims[i]->intersect(ray, tmp)) {
+            ++hits;
+            if ((tmp.t < rec.t) && (tmp.t > 0.0f)) {
+                rec = tmp;
+                rec.prim_id = (uint32_t)i;
+                result = true;
+            }
+        }
+    }
+    return result;
+}
+
+// ---------------------------------------------------------------------------
+// Random draws (replaces core/prng.h; see include/spt.h for the definition)
+// ---------------------------------------------------------------------------
+static inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static const uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+
+struct Rng {
+    uint64_t key = 0;
+    uint32_t k = 0;
+    const float* stream = nullptr;
+    uint32_t stride = 0;
+    bool overflow = false;
+    void init(uint64_t seed, uint64_t path) {
+        key = mix64(seed ^ (path * kGolden));
+        k = 0;
+    }
+    float operator()() {
+        if (stream) {
+            if (k >= stride) {
+                overflow = true;
+                return 0.0f;
+            }
+            return stream[k++];
+        }
+        uint64_t z = mix64(key + (uint64_t)(k + 1) * kGolden);
+        ++k;
+        return (float)(z >> 40) * (1.0f / 16777216.0f);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Cameras  (cameras/camera.cpp, pinhole_camera.cpp, orthographic_camera.cpp)
+// ---------------------------------------------------------------------------
+struct Camera {
+    uint32_t type = SPT_CAMERA_PINHOLE;
+    uint32_t width = 0, height = 0;
+    V3 position;
+    ONB onb;
+    V3 ulc;               // local upper-left corner
+    float pdx = 0, pdy = 0;  // orthographic pixel deltas
+    V3 ortho_dir;
+    explicit Camera(const spt_camera& c) {
+        type = c.type;
+        width = c.width;
+        height = c.height;
+        position = V3(c.position[0], c.position[1], c.position[2]);
+        V3 look_at(c.look_at[0], c.look_at[1], c.look_at[2]);
+        V3 up(c.up[0], c.up[1], c.up[2]);
+        V3 direction = normalize(look_at - position);                 // camera.cpp:9
+        onb = ONB::fromUW(normalize(cross(up, -direction)), -direction);  // camera.cpp:10
+        if (type == SPT_CAMERA_PINHOLE) {                              // pinhole_camera.cpp:3-12
+            float a = width * 0.5f;
+            float b = (float)(height * 0.5);
+            // tan() receives a float; the unqualified call resolves to ::tan(double),
+            // so the quotient is formed in double and narrowed (assumption, see header)
+            float vpd = (float)((double)b / std::tan((double)(c.fov_deg * 0.5f * PI_f / 180.0f)));
+            ulc = V3(-a, b, -vpd);
+        } else {  // orthographic_camera.cpp:5-19
+            float ww = c.max_x - c.min_x, wh = c.max_y - c.min_y;
+            ulc = V3(c.min_x, c.max_y, 0.0f);
+            pdx = ww / width;
+            pdy = wh / height;
+            ortho_dir = mul(onb.l2w, V3(0.0f, 0.0f, -1.0f));
+        }
+    }
+    Ray getRay(float x, float y) const {
+        if (type == SPT_CAMERA_PINHOLE) {  // pinhole_camera.cpp:14-18
+            V3 ld = normalize(ulc + V3(x, -y, 0.0f));
+            return Ray{position, mul(onb.l2w, ld)};
+        }
+        V3 lo = ulc + V3(x * pdx, y * -pdy, 0.0f);  // orthographic_camera.cpp:21-25
+        return Ray{mul(onb.l2w, lo) + position, ortho_dir};
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Pixel samplers  (pixel_samplers/*.cpp). getSample() is called twice per
+// camera sample (pathtracer.cpp:110-111): x from the first call, y from the
+// second. The order of the two draws inside one call is unspecified in C++;
+// here the first draw goes to x.
+// ---------------------------------------------------------------------------
+struct PixelSampler {
+    uint32_t type;
+    uint32_t n;    // subpixels per side
+    uint32_t spp;  // effective spp
+    float w;       // subpixel width
+    PixelSampler(uint32_t t, uint32_t requested) : type(t) {
+        if (t == SPT_PIXEL_UNIFORM) {
+            n = 1;
+            spp = requested;
+            w = 1.0f;
+        } else {
+            n = (uint32_t)std::ceil(std::sqrt((double)requested));
+            spp = n * n;
+            w = 1.0f / n;
+        }
+    }
+    // returns (sx, sy); 0 (regular) or 4 draws (jittered, uniform)
+    void sample(uint32_t s, Rng& rng, float& sx, float& sy) const {
+        if (type == SPT_PIXEL_REGULAR) {  // regular_pixel_sampler.cpp:3-19
+            float off = w * 0.5f;
+            uint32_t i = s / n, j = s % n;
+            sx = w * i + off;
+            sy = w * j + off;
+        } else if (type == SPT_PIXEL_JITTERED) {  // jittered_pixel_sampler.cpp:3-20
+            uint32_t i = s / n, j = s % n;
+            float bx = w * i, by = w * j;
+            float u0 = rng(), u1 = rng();
+            (void)u1;
+            sx = bx + w * u0;
+            float u2 = rng(), u3 = rng();
+            (void)u2;
+            sy = by + w * u3;
+        } else {  // uniform_pixel_sampler.cpp:7-10
+            float u0 = rng(), u1 = rng();
+            (void)u1;
+            float u2 = rng(), u3 = rng();
+            (void)u2;
+            sx = u0;
+            sy = u3;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// BSDF layers  (materials/*.cpp, bsdf_samplers/*.cpp)
+// ---------------------------------------------------------------------------
+struct Sample {
+    V3 wi;
+    float pdf;
+};
+
+// fresnel_schlick.cpp:11-18 (reflectance at normal incidence)
+static V3 schlickR0(V3 r0, float cos_theta) {
+    float c5 = 1.0f - cos_theta;
+    c5 = c5 * c5 * c5 * c5 * c5;
+    return r0 + (1.0f - r0) * c5;
+}
+// fresnel_schlick.cpp:20-38 (two IORs)
+static V3 schlickIOR(float ior_i_, float ior_t_, float cos_theta) {
+    float ior_i = ior_i_, ior_t = ior_t_;
+    if (cos_theta < 0.0f) {
+        ior_i = ior_t_;
+        ior_t = ior_i_;
+        float eta = ior_i / ior_t;
+        float a = 1 - eta * eta * (1.0f - cos_theta * cos_theta);
+        cos_theta = std::sqrt(a);
+    }
+    float r0 = (ior_i - ior_t) / (ior_i + ior_t);
+    r0 = r0 * r0;
+    float c5 = 1.0f - cos_theta;
+    c5 = c5 * c5 * c5 * c5 * c5;
+    return V3(r0 + (1.0f - r0) * c5);
+}
+// Exact Fresnel reflectance of a conductor with complex IOR eta + i k, seen
+// from a medium of IOR 1 (no reference counterpart; textbook form, e.g. Pharr,
+// Jakob, Humphreys, "Physically Based Rendering", 3rd ed., sec. 8.2.1).
+static float fresnelConductor1(float cos_i, float eta, float k) {
+    cos_i = gmin(gmax(cos_i, -1.0f), 1.0f);
+    float c2 = cos_i * cos_i;
+    float s2 = 1.0f - c2;
+    float e2 = eta * eta, k2 = k * k;
+    float t0 = e2 - k2 - s2;
+    float a2b2 = std::sqrt(t0 * t0 + 4.0f * e2 * k2);
+    float t1 = a2b2 + c2;
+    float a = std::sqrt(0.5f * (a2b2 + t0));
+    float t2 = 2.0f * cos_i * a;
+    float rs = (t1 - t2) / (t1 + t2);
+    float t3 = c2 * a2b2 + s2 * s2;
+    float t4 = t2 * s2;
+    float rp = rs * (t3 - t4) / (t3 + t4);
+    return 0.5f * (rp + rs);
+}
+static V3 fresnelConductor(float cos_i, const float* eta, const float* k) {
+    return V3(fresnelConductor1(cos_i, eta[0], k[0]), fresnelConductor1(cos_i, eta[1], k[1]),
+              fresnelConductor1(cos_i, eta[2], k[2]));
+}
+
+// GGX (Walter, Marschner, Li, Torrance 2007) normal distribution and Smith G1.
+static float ggxD(float cos_h, float alpha) {
+    if (cos_h <= 0.0f) return 0.0f;
+    float a2 = alpha * alpha;
+    float c2 = cos_h * cos_h;
+    float den = c2 * (a2 - 1.0f) + 1.0f;
+    return a2 / (PI_f * den * den);
+}
+static float ggxG1(float cos_v, float alpha) {
+    if (cos_v <= 0.0f) return 0.0f;
+    float a2 = alpha * alpha;
+    float c2 = cos_v * cos_v;
+    return 2.0f * cos_v / (cos_v + std::sqrt(a2 + (1.0f - a2) * c2));
+}
+
+struct Layer {
+    spt_layer p;
+    bool opaque() const { return p.type != SPT_BSDF_SMOOTH_DIELECTRIC; }
+
+    V3 f(V3 wo, V3 wi) const {
+        switch (p.type) {
+            case SPT_BSDF_LAMBERTIAN: {  // lambertian.cpp:9-16
+                V3 brdf = V3(p.kd[0], p.kd[1], p.kd[2]) / PI_f;
+                return (wi.y > 0.0f) ? brdf : V3(0.0f);
+            }
+            case SPT_BSDF_COOK_TORRANCE: {  // cook_torrance.cpp:9-44
+                if (wi.y <= 0.0f) return V3(0.0f);
+                V3 h = normalize(wo + wi);
+                float nv = smax(0.0f, wo.y), nl = smax(0.0f, wi.y);
+                float vh = smax(0.0f, dot(wo, h)), nh = smax(0.0f, h.y);
+                float m = p.roughness;
+                if ((nv == 0.0f) || (nl == 0.0f) || (vh == 0.0f) || (nh == 0.0f) || (m == 0.0f)) return V3(0.0f);
+                float lh = smax(0.0f, dot(wi, h));
+                float nh2 = nh * nh;
+                float m2 = m * m;
+                float d1 = 1.0f / (PI_f * m2 * nh2 * nh2);
+                float d2 = (nh2 - 1.0f) / (m2 * nh2);
+                // unqualified exp(float) resolves to ::exp(double) in this file (assumption)
+                float d = (float)((double)d1 * std::exp((double)d2));
+                float g1 = 2.0f * nh / vh;
+                float g = smin(1.0f, smin(g1 * nv, g1 * nl));
+                V3 F = schlickR0(V3(p.r0[0], p.r0[1], p.r0[2]), lh);
+                return (F * d * g) / (4.0f * nv * nl);
+            }
+            case SPT_BSDF_SMOOTH_CONDUCTOR: {  // smooth_conductor.cpp:6-14
+                if (wi.y > 0.0f) return schlickR0(V3(p.r0[0], p.r0[1], p.r0[2]), wi.y) / wi.y;
+                return V3(0.0f);
+            }
+            case SPT_BSDF_SMOOTH_DIELECTRIC: {  // smooth_dielectric.cpp:15-21
+                if (wi.y == 0.0f) return V3(0.0f);
+                return V3(1.0f) / wi.y;
+            }
+            case SPT_BSDF_GGX_CONDUCTOR: {
+                if (wi.y <= 0.0f || wo.y <= 0.0f) return V3(0.0f);
+                V3 h = normalize(wo + wi);
+                float D = ggxD(h.y, p.roughness);
+                float G = ggxG1(wo.y, p.roughness) * ggxG1(wi.y, p.roughness);
+                V3 F = fresnelConductor(dot(wi, h), p.eta, p.k);
+                return F * (D * G / (4.0f * wo.y * wi.y));
+            }
+        }
+        return V3(0.0f);
+    }
+
+    V3 reflectance(V3 wo, V3 wi) const {
+        switch (p.type) {
+            case SPT_BSDF_LAMBERTIAN:
+                return V3(p.kd[0], p.kd[1], p.kd[2]);  // lambertian.cpp:18-22
+            case SPT_BSDF_COOK_TORRANCE: {             // cook_torrance.cpp:46-51
+                V3 h = normalize(wo + wi);
+                float lh = smax(0.0f, dot(wi, h));
+                return schlickR0(V3(p.r0[0], p.r0[1], p.r0[2]), lh);
+            }
+            case SPT_BSDF_SMOOTH_CONDUCTOR:  // smooth_conductor.cpp:16-20
+                return schlickR0(V3(p.r0[0], p.r0[1], p.r0[2]), wi.y);
+            case SPT_BSDF_SMOOTH_DIELECTRIC:
+                return V3(1.0f);  // smooth_dielectric.cpp:23-28
+            case SPT_BSDF_GGX_CONDUCTOR: {
+                V3 h = normalize(wo + wi);
+                return fresnelConductor(smax(0.0f, dot(wi, h)), p.eta, p.k);
+            }
+        }
+        return V3(0.0f);
+    }
+
+    // uniform_bsdf_sampler.cpp:5-14
+    static Sample uniformSample(Rng& rng) {
+        float r1 = rng();
+        float r2 = rng();
+        float phi = 2.0f * PI_f * r2;
+        float s = sqrtf(1.0f - r1 * r1);
+        return Sample{V3(std::cos(phi) * s, r1, std::sin(phi) * s), 1.0f / (2.0f * PI_f)};
+    }
+    // smooth_specular_bsdf_sampler.cpp:5-7
+    static Sample specular(V3 wo) { return Sample{V3(-wo.x, wo.y, -wo.z), 1.0f}; }
+
+    // cook_torrance_bsdf_sampler.cpp:22-42
+    float ctProbability(V3 wo, V3 wi) const {
+        if (wi.y <= 0.0f) return 1.0f;
+        V3 h = normalize(wo + wi);
+        float nh = smax(0.0f, h.y);
+        if (nh == 0.0f) return 1e-5f;
+        float nh_2 = nh * nh;
+        float m2 = p.roughness * p.roughness;
+        float d1 = 1.0f / (PI_f * m2 * nh * nh * nh * nh);
+        float d2 = (nh_2 - 1.0f) / (m2 * nh_2);
+        float om = dot(wi, h);
+        if (om <= 0.0f) return 1e+5f;
+        return (d1 * std::exp(d2) * nh) / (4.0f * om);  // float overload via <math.h>
+    }
+
+    float ggxPdf(V3 wo, V3 wi) const {
+        if (wi.y <= 0.0f || wo.y <= 0.0f) return 0.0f;
+        V3 h = normalize(wo + wi);
+        float om = dot(wi, h);
+        if (om <= 0.0f) return 0.0f;
+        return ggxD(h.y, p.roughness) * h.y / (4.0f * om);
+    }
+
+    Sample sample(V3 wo, Rng& rng) const {
+        switch (p.type) {
+            case SPT_BSDF_LAMBERTIAN: {
+                if (p.sampler == SPT_SAMPLER_UNIFORM) return uniformSample(rng);
+                // cosine_bsdf_sampler.cpp:5-17
+                float r1 = rng();
+                float r2 = rng();
+                float phi = 2.0f * PI_f * r2;
+                float cos_theta = sqrtf(r1);
+                float s = sqrtf(1.0f - cos_theta * cos_theta);
+                return Sample{V3(std::cos(phi) * s, cos_theta, std::sin(phi) * s), cos_theta / PI_f};
+            }
+            case SPT_BSDF_COOK_TORRANCE: {
+                if (p.sampler == SPT_SAMPLER_UNIFORM) return uniformSample(rng);
+                // cook_torrance_bsdf_sampler.cpp:7-20 (Beckmann)
+                float r1 = rng();
+                float r2 = rng();
+                float phi = 2.0f * PI_f * r2;
+                float theta = std::atan(sqrtf(-(p.roughness * p.roughness) * std::log(1.0f - r1)));
+                V3 m = V3(std::cos(phi) * std::sin(theta), std::cos(theta), std::sin(phi) * std::sin(theta));
+                V3 wi = 2.0f * m * dot(m, wo) - wo;
+                return Sample{wi, ctProbability(wo, wi)};
+            }
+            case SPT_BSDF_SMOOTH_CONDUCTOR:
+                if (p.sampler == SPT_SAMPLER_UNIFORM) return uniformSample(rng);
+                return specular(wo);
+            case SPT_BSDF_SMOOTH_DIELECTRIC: {  // smooth_dielectric.cpp:30-51
+                float ior_i = 1.0f, ior_t = p.ior;
+                if (wo.y < 0.0f) {
+                    ior_i = p.ior;
+                    ior_t = 1.0f;
+                }
+                float a = ((ior_i * ior_i) / (ior_t * ior_t)) * (1.0f - wo.y * wo.y);
+                if (a > 1.0f) return specular(wo);
+                V3 F = schlickIOR(1.0f, p.ior, wo.y);
+                float maxr = smax(smax(F.x, F.y), F.z);
+                if (rng() < maxr) return specular(wo);
+                // refraction_bsdf_sampler.cpp:5-24
+                V3 n(0.0f, 1.0f, 0.0f);
+                float ri = 1.0f, rt = p.ior, signal = 1.0f;
+                if (wo.y < 0.0f) {
+                    ri = p.ior;
+                    rt = 1.0f;
+                    signal = -1.0f;
+                }
+                float b = 1.0f - ((ri * ri) / (rt * rt)) * (1.0f - wo.y * wo.y);
+                V3 wi = normalize((ri / rt) * (n * wo.y - wo) - signal * n * sqrtf(b));
+                return Sample{wi, 1.0f};
+            }
+            case SPT_BSDF_GGX_CONDUCTOR: {
+                if (p.sampler == SPT_SAMPLER_UNIFORM) return uniformSample(rng);
+                // D(m) cos(theta_m) sampling (Walter et al. 2007, eq. 35-36)
+                float r1 = rng();
+                float r2 = rng();
+                float phi = 2.0f * PI_f * r2;
+                float a2 = p.roughness * p.roughness;
+                float tan2 = a2 * r1 / (1.0f - r1);
+                float cos_m = 1.0f / sqrtf(1.0f + tan2);
+                float sin_m = sqrtf(smax(0.0f, 1.0f - cos_m * cos_m));
+                V3 m = V3(std::cos(phi) * sin_m, cos_m, std::sin(phi) * sin_m);
+                V3 wi = 2.0f * m * dot(m, wo) - wo;
+                float pdf = 1.0f;
+                if (wi.y > 0.0f) {
+                    float om = dot(wi, m);
+                    pdf = (om > 0.0f) ? ggxD(cos_m, p.roughness) * cos_m / (4.0f * om) : 1.0f;
+                }
+                return Sample{wi, pdf};
+            }
+        }
+        return Sample{V3(0.0f, 1.0f, 0.0f), 1.0f};
+    }
+
+    // density of sample() for a given direction (layers with a density only)
+    float pdf(V3 wo, V3 wi) const {
+        switch (p.type) {
+            case SPT_BSDF_LAMBERTIAN:
+                if (p.sampler == SPT_SAMPLER_UNIFORM) return (wi.y > 0.0f) ? 1.0f / (2.0f * PI_f) : 0.0f;
+                return (wi.y > 0.0f) ? wi.y / PI_f : 0.0f;
+            case SPT_BSDF_COOK_TORRANCE:
+                return ctProbability(wo, wi);
+            case SPT_BSDF_GGX_CONDUCTOR:
+                return ggxPdf(wo, wi);
+        }
+        return 0.0f;
+    }
+};
+
+struct Material {
+    std::vector<Layer> layers;
+    V3 emission;
+    // layered_bsdf.cpp:10-74
+    void evalBSDF(V3 wo, V3& bsdf, V3& refl, float& pdf, V3& wi, Rng& rng) const {
+        const long n = (long)layers.size();
+        long id;
+        V3 fb(1.0f), fr(1.0f);
+        float fp = 1.0f;
+        V3 nwo = wo;
+        bool reached_inf = false;
+        int bounce = 0;
+        id = (nwo.y > 0.0f) ? 0 : n - 1;
+        while ((id > -1) && (id < n) && (fp > 0.0f) && (!reached_inf) && (bounce < 500)) {
+            Sample s = layers[id].sample(nwo, rng);
+            wi = s.wi;
+            fp *= s.pdf;
+            if (!(fp < kInf)) reached_inf = true;
+            if (wi.y > 0.0f) {
+                if (id == 0)
+                    fb = fb * layers[id].f(nwo, wi);
+                else
+                    fb = fb * (layers[id].f(nwo, wi) * wi.y);
+                fr = fr * layers[id].reflectance(nwo, wi);
+                --id;
+            } else if ((wi.y < 0.0f) && !layers[id].opaque()) {
+                if (id == n - 1)
+                    fb = fb * layers[id].f(nwo, wi);
+                else
+                    fb = fb * (layers[id].f(nwo, wi) * wi.y);
+                fr = fr * layers[id].reflectance(nwo, wi);
+                ++id;
+            } else {
+                fp = 0.0f;
+            }
+            if (!((fb.x < kInf) && (fb.y < kInf) && (fb.z < kInf))) reached_inf = true;
+            nwo = -wi;
+            ++bounce;
+        }
+        if ((bounce < 500) && !reached_inf && (fp > 0.0f)) {
+            pdf = fp;
+            bsdf = fb;
+            refl = fr;
+        } else {
+            pdf = 1.0f;
+            bsdf = V3(0.0f);
+            refl = V3(0.0f);
+        }
+    }
+};
+
+static Material makeMaterial(const spt_material& m) {
+    Material out;
+    out.emission = V3(m.emission[0], m.emission[1], m.emission[2]);
+    for (uint32_t i = 0; i < m.n_layers && i < SPT_MAX_LAYERS; ++i) out.layers.push_back(Layer{m.layers[i]});
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Scene + integrators
+// ---------------------------------------------------------------------------
+struct Scene {
+    int accel = 1;
+    std::vector<std::unique_ptr<Primitive>> prims;
+    SBVH bvh;
+    std::vector<Material> materials;
+    bool built = false;
+
+    bool intersect(const Ray& ray, Rec& rec, std::size_t& tests, std::size_t& hits) const {
+        if (accel == 0) return bruteIntersect(prims, ray, rec, tests, hits);
+        if (!bvh.root) {
+            rec.t = kInf;
+            tests = hits = 0;
+            return false;
+        }
+        return bvh.intersect(ray, rec, tests, hits);
+    }
+};
+
+struct PTParams {
+    int max_depth, rr_depth;
+    V3 background;
+};
+
+// integrators/pathtracer.cpp:12-57
+static V3 traceRay(const Scene& sc, const PTParams& pp, const Ray& ray, int depth, Rng& rng) {
+    Rec rec;
+    rec.t = kInf;
+    if ((pp.max_depth >= 0) && (depth >= pp.max_depth)) return V3(0.0f);
+    std::size_t tests, hits;
+    if (!sc.intersect(ray, rec, tests, hits)) return depth == 0 ? pp.background : V3(0.0f);
+    ONB frame = ONB::fromUp(rec.normal);
+    V3 wo = normalize(mul(frame.w2l, -ray.d));
+    V3 wi, bsdf, refl;
+    float pdf;
+    const Material& mat = sc.materials[rec.material_id];
+    mat.evalBSDF(wo, bsdf, refl, pdf, wi, rng);
+    V3 world_wi = mul(frame.l2w, wi);
+    Ray next{rec.pos + world_wi * 1e-3f, world_wi};
+    V3 emission = mat.emission;
+    float k = smax(smax(refl.x, refl.y), refl.z);
+    if ((pp.rr_depth >= 0) && (depth >= pp.rr_depth)) {
+        if ((rng() < k) && (depth < 1000))
+            bsdf = bsdf / k;
+        else
+            return emission;
+    }
+    return emission + (bsdf * traceRay(sc, pp, next, depth + 1, rng) * wi.y) / pdf;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+struct orc_scene {
+    Scene sc;
+};
+
+extern "C" {
+
+orc_scene* orc_scene_create(int accel) {
+    orc_scene* s = new orc_scene();
+    s->sc.accel = accel;
+    s->sc.bvh.prims = &s->sc.prims;
+    return s;
+}
+
+void orc_scene_destroy(orc_scene* s) { delete s; }
+
+int orc_scene_add_triangles(orc_scene* s, const float* v, const float* n, const uint32_t* m, uint32_t count) {
+    if (!s || (!v && count)) return SPT_ERR_INVALID;
+    for (uint32_t i = 0; i < count; ++i) {
+        const float* p = v + 9ull * i;
+        V3 a(p[0], p[1], p[2]), b(p[3], p[4], p[5]), c(p[6], p[7], p[8]);
+        uint32_t mat = m ? m[i] : 0;
+        const float* q = n ? n + 9ull * i : nullptr;
+        bool smooth = false;
+        if (q)
+            for (int j = 0; j < 9; ++j) smooth = smooth || (q[j] != 0.0f);
+        if (smooth) {
+            s->sc.prims.emplace_back(new SmoothTriangle(a, b, c, V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5]),
+                                                        V3(q[6], q[7], q[8]), mat));
+        } else {
+            s->sc.prims.emplace_back(new Triangle(a, b, c, mat));
+        }
+    }
+    return SPT_OK;
+}
+
+int orc_scene_add_spheres(orc_scene* s, const float* sp, const uint32_t* m, uint32_t count) {
+    if (!s || (!sp && count)) return SPT_ERR_INVALID;
+    for (uint32_t i = 0; i < count; ++i)
+        s->sc.prims.emplace_back(
+            new Sphere(V3(sp[4 * i], sp[4 * i + 1], sp[4 * i + 2]), sp[4 * i + 3], m ? m[i] : 0));
+    return SPT_OK;
+}
+
+int orc_scene_build(orc_scene* s) {
+    if (!s) return SPT_ERR_INVALID;
+    try {
+        if (s->sc.accel == 1) s->sc.bvh.build();
+    } catch (const std::exception&) {
+        return SPT_ERR_INVALID;
+    }
+    s->sc.built = true;
+    return SPT_OK;
+}
+
+int orc_set_materials(orc_scene* s, const spt_material* mats, uint32_t n) {
+    if (!s || (!mats && n)) return SPT_ERR_INVALID;
+    s->sc.materials.clear();
+    for (uint32_t i = 0; i < n; ++i) s->sc.materials.push_back(makeMaterial(mats[i]));
+    return SPT_OK;
+}
+
+int orc_bvh_stats(const orc_scene* s, uint64_t* out) {
+    if (!s || !out) return SPT_ERR_INVALID;
+    const BVHStats& st = s->sc.bvh.stats;
+    out[0] = st.nodes;
+    out[1] = st.inner;
+    out[2] = st.leaves;
+    out[3] = st.height;
+    out[4] = st.refs;
+    out[5] = st.min_leaf;
+    out[6] = st.max_leaf;
+    out[7] = st.unreferenced;
+    return SPT_OK;
+}
+
+int orc_bvh_dump(const orc_scene* s, const char* path) {
+    if (!s || !path) return SPT_ERR_INVALID;
+    return s->sc.bvh.dump(path) ? SPT_OK : SPT_ERR_INVALID;
+}
+
+int orc_bvh_leaf_order(const orc_scene* s, uint32_t* ids) {
+    if (!s || !ids) return SPT_ERR_INVALID;
+    std::vector<uint32_t> order;
+    s->sc.bvh.leafOrder(s->sc.bvh.root.get(), order);
+    std::memcpy(ids, order.data(), order.size() * sizeof(uint32_t));
+    return SPT_OK;
+}
+
+void orc_trace(const orc_scene* s, const spt_ray32* rays, uint64_t n, spt_hit* hits, uint32_t* tests,
+               uint32_t* isects, float* normals, int n_threads) {
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1024)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const spt_ray32& r = rays[i];
+        Ray ray{V3(r.ox, r.oy, r.oz), V3(r.dx, r.dy, r.dz)};
+        Rec rec;
+        std::size_t nt = 0, nh = 0;
+        bool hit = s->sc.intersect(ray, rec, nt, nh);
+        if (hits) {
+            hits[i].t = hit ? rec.t : kInf;
+            hits[i].prim_id = hit ? rec.prim_id : SPT_MISS;
+        }
+        if (tests) tests[i] = (uint32_t)nt;
+        if (isects) isects[i] = (uint32_t)nh;
+        if (normals) {
+            normals[3 * i + 0] = hit ? rec.normal.x : 0.0f;
+            normals[3 * i + 1] = hit ? rec.normal.y : 0.0f;
+            normals[3 * i + 2] = hit ? rec.normal.z : 0.0f;
+        }
+    }
+}
+
+int orc_camera_ray(const spt_camera* cam, float x, float y, spt_ray32* out) {
+    if (!cam || !out) return SPT_ERR_INVALID;
+    Camera c(*cam);
+    Ray r = c.getRay(x, y);
+    *out = spt_ray32{r.o.x, r.o.y, r.o.z, 0.0f, r.d.x, r.d.y, r.d.z, 0.0f};
+    return SPT_OK;
+}
+
+int orc_pixel_sample(uint32_t sampler, uint32_t spp, uint32_t sample, uint64_t seed, uint64_t path, float* sx,
+                     float* sy) {
+    PixelSampler ps(sampler, spp);
+    Rng rng;
+    rng.init(seed, path);
+    ps.sample(sample, rng, *sx, *sy);
+    return (int)rng.k;
+}
+
+float orc_rng_u(uint64_t seed, uint64_t path, uint32_t k) {
+    Rng rng;
+    rng.init(seed, path);
+    rng.k = k;
+    return rng();
+}
+
+int orc_material_sample(const spt_material* m, const float* wo, uint64_t seed, uint64_t path, float* out) {
+    if (!m || !wo || !out) return SPT_ERR_INVALID;
+    Material mat = makeMaterial(*m);
+    Rng rng;
+    rng.init(seed, path);
+    V3 bsdf, refl, wi;
+    float pdf;
+    mat.evalBSDF(V3(wo[0], wo[1], wo[2]), bsdf, refl, pdf, wi, rng);
+    float o[11] = {wi.x, wi.y, wi.z, pdf, bsdf.x, bsdf.y, bsdf.z, refl.x, refl.y, refl.z, (float)rng.k};
+    std::memcpy(out, o, sizeof(o));
+    return SPT_OK;
+}
+
+int orc_layer_eval(const spt_layer* l, const float* wo, const float* wi, float* out) {
+    if (!l || !wo || !wi || !out) return SPT_ERR_INVALID;
+    Layer layer{*l};
+    V3 a(wo[0], wo[1], wo[2]), b(wi[0], wi[1], wi[2]);
+    V3 f = layer.f(a, b);
+    out[0] = f.x;
+    out[1] = f.y;
+    out[2] = f.z;
+    out[3] = layer.pdf(a, b);
+    return SPT_OK;
+}
+
+// integrators/pathtracer.cpp:59-147 (render loop, OpenMP over rows)
+int orc_render(const orc_scene* s, const spt_camera* cam, const spt_render_params* p, float* rgb32, double* rgb64,
+               float* path_rgb, uint64_t* overflows, int n_threads) {
+    if (!s || !cam || !p) return SPT_ERR_INVALID;
+    if (s->sc.materials.empty()) return SPT_ERR_STATE;
+    Camera camera(*cam);
+    PixelSampler ps(p->pixel_sampler, p->spp);
+    const uint32_t spp = ps.spp;
+    const uint32_t s_begin = p->spp_begin;
+    const uint32_t s_count = p->spp_count ? p->spp_count : spp - s_begin;
+    if (s_begin + s_count > spp) return SPT_ERR_INVALID;
+    uint32_t vt = p->viewport[0], vl = p->viewport[1], vw = p->viewport[2], vh = p->viewport[3];
+    if (vw == 0) {
+        vt = 0;
+        vl = 0;
+        vw = cam->width;
+        vh = cam->height;
+    }
+    if (vl + vw > cam->width || vt + vh > cam->height) return SPT_ERR_INVALID;
+    if (p->rng == SPT_RNG_STREAM && (!p->d_stream || p->stream_stride == 0)) return SPT_ERR_INVALID;
+    PTParams pp{p->max_depth, p->rr_depth, V3(p->background[0], p->background[1], p->background[2])};
+    uint64_t n_over = 0;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : n_over)
+#endif
+    for (int64_t yy = 0; yy < (int64_t)vh; ++yy) {
+        const uint32_t y = vt + (uint32_t)yy;
+        for (uint32_t x = vl; x < vl + vw; ++x) {
+            V3 sum32(0.0f);
+            double sum64[3] = {0.0, 0.0, 0.0};
+            const uint64_t pix_local = (uint64_t)(y - vt) * vw + (x - vl);
+            for (uint32_t smp = s_begin; smp < s_begin + s_count; ++smp) {
+                const uint64_t path = ((uint64_t)y * cam->width + x) * spp + smp;
+                const uint64_t local = pix_local * s_count + (smp - s_begin);
+                Rng rng;
+                if (p->rng == SPT_RNG_STREAM) {
+                    rng.stream = p->d_stream + local * p->stream_stride;
+                    rng.stride = p->stream_stride;
+                } else {
+                    rng.init(p->seed, path);
+                }
+                float sx, sy;
+                ps.sample(smp, rng, sx, sy);
+                float xs = x + sx;  // pathtracer.cpp:110-111
+                float ys = y + sy;
+                Ray ray = camera.getRay(xs, ys);
+                V3 L = traceRay(s->sc, pp, ray, 0, rng);
+                if (rng.overflow) ++n_over;
+                sum32 = sum32 + L;  // pathtracer.cpp:117
+                sum64[0] += L.x;
+                sum64[1] += L.y;
+                sum64[2] += L.z;
+                if (path_rgb) {
+                    path_rgb[3 * local + 0] = L.x;
+                    path_rgb[3 * local + 1] = L.y;
+                    path_rgb[3 * local + 2] = L.z;
+                }
+            }
+            const uint64_t pix = (uint64_t)y * cam->width + x;
+            if (rgb32) {  // pathtracer.cpp:120-121
+                V3 v = sum32 / static_cast<float>(s_count);
+                rgb32[3 * pix + 0] = v.x;
+                rgb32[3 * pix + 1] = v.y;
+                rgb32[3 * pix + 2] = v.z;
+            }
+            if (rgb64) {
+                for (int c = 0; c < 3; ++c) rgb64[3 * pix + c] = sum64[c] / (double)s_count;
+            }
+        }
+    }
+    if (overflows) *overflows = n_over;
+    return SPT_OK;
+}
+
+// integrators/raycaster.h:61-99 with normal_raycaster.cpp:19-31,
+// depth_raycaster.cpp:36-48, int_test_count_raycaster.cpp:75-85
+int orc_raycast(const orc_scene* s, const spt_camera* cam, uint32_t kind, void* out, int n_threads) {
+    if (!s || !cam || !out || kind > 2) return SPT_ERR_INVALID;
+    Camera camera(*cam);
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int64_t yy = 0; yy < (int64_t)cam->height; ++yy) {
+        const uint32_t y = (uint32_t)yy;
+        for (uint32_t x = 0; x < cam->width; ++x) {
+            float xs = x + 0.5f, ys = y + 0.5f;  // RegularPixelSampler(1): (0.5, 0.5)
+            Ray ray = camera.getRay(xs, ys);
+            Rec rec;
+            std::size_t nt = 0, nh = 0;
+            bool hit = s->sc.intersect(ray, rec, nt, nh);
+            const uint64_t pix = (uint64_t)y * cam->width + x;
+            if (kind == SPT_RAYCAST_NORMAL) {
+                float* o = (float*)out + 3 * pix;
+                o[0] = hit ? rec.normal.x : 0.0f;
+                o[1] = hit ? rec.normal.y : 0.0f;
+                o[2] = hit ? rec.normal.z : 0.0f;
+            } else if (kind == SPT_RAYCAST_DEPTH) {
+                ((float*)out)[pix] = hit ? rec.t : -1.0f;
+            } else {
+                ((uint32_t*)out)[pix] = (uint32_t)nt;
+            }
+        }
+    }
+    return SPT_OK;
+}
+
+}  // extern "C"

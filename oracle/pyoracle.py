@@ -1,0 +1,199 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    import swpathtracer_amd._lib as L  # POD struct layouts (include/spt.h)
+    lib = C.CDLL(LIB_PATH)
+    vp, u32, u64, f, i = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_int
+    protos = {
+        "orc_scene_create": (vp, [i]),
+        "orc_scene_destroy": (None, [vp]),
+        "orc_scene_add_triangles": (i, [vp, vp, vp, vp, u32]),
+        "orc_scene_add_spheres": (i, [vp, vp, vp, u32]),
+        "orc_scene_build": (i, [vp]),
+        "orc_set_materials": (i, [vp, C.POINTER(L.Material), u32]),
+        "orc_bvh_stats": (i, [vp, vp]),
+        "orc_bvh_dump": (i, [vp, C.c_char_p]),
+        "orc_bvh_leaf_order": (i, [vp, vp]),
+        "orc_trace": (None, [vp, vp, u64, vp, vp, vp, vp, i]),
+        "orc_camera_ray": (i, [C.POINTER(L.Camera), f, f, vp]),
+        "orc_pixel_sample": (i, [u32, u32, u32, u64, u64, C.POINTER(f), C.POINTER(f)]),
+        "orc_rng_u": (f, [u64, u64, u32]),
+        "orc_material_sample": (i, [C.POINTER(L.Material), vp, u64, u64, vp]),
+        "orc_layer_eval": (i, [C.POINTER(L.Layer), vp, vp, vp]),
+        "orc_render": (i, [vp, C.POINTER(L.Camera), C.POINTER(L.RenderParams), vp, vp, vp, vp, i]),
+        "orc_raycast": (i, [vp, C.POINTER(L.Camera), u32, vp, i]),
+    }
+    for name, (res, args) in protos.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Scene:
+    """Reference-semantics scene: primitives in insertion order + SBVH or brute force."""
+
+    def __init__(self, verts=None, normals=None, mat_ids=None, spheres=None, sphere_mats=None, accel=1,
+                 materials=None):
+        self.lib = load()
+        self.h = self.lib.orc_scene_create(accel)
+        self.n = 0
+        if verts is not None:
+            v = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
+            n = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+            m = None if mat_ids is None else np.ascontiguousarray(mat_ids, np.uint32)
+            assert self.lib.orc_scene_add_triangles(self.h, _p(v), _p(n), _p(m), v.shape[0]) == 0
+            self.n += v.shape[0]
+        if spheres is not None:
+            s = np.ascontiguousarray(spheres, np.float32).reshape(-1, 4)
+            m = None if sphere_mats is None else np.ascontiguousarray(sphere_mats, np.uint32)
+            assert self.lib.orc_scene_add_spheres(self.h, _p(s), _p(m), s.shape[0]) == 0
+            self.n += s.shape[0]
+        assert self.lib.orc_scene_build(self.h) == 0
+        if materials is not None:
+            self.set_materials(materials)
+
+    def set_materials(self, mats):
+        import swpathtracer_amd._lib as L
+        arr = (L.Material * len(mats))(*mats)
+        assert self.lib.orc_set_materials(self.h, arr, len(mats)) == 0
+
+    def close(self):
+        if self.h:
+            self.lib.orc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self) -> dict:
+        out = np.zeros(8, np.uint64)
+        self.lib.orc_bvh_stats(self.h, _p(out))
+        keys = ["nodes", "inner", "leaves", "height", "references", "min_leaf", "max_leaf", "unreferenced"]
+        return dict(zip(keys, [int(x) for x in out]))
+
+    def dump(self, path):
+        assert self.lib.orc_bvh_dump(self.h, path.encode()) == 0
+
+    def leaf_order(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint32)
+        self.lib.orc_bvh_leaf_order(self.h, _p(out))
+        return out
+
+    def trace(self, rays: np.ndarray, threads: int = 0, normals: bool = False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.zeros((n, 2), np.uint32)
+        tests = np.zeros(n, np.uint32)
+        isects = np.zeros(n, np.uint32)
+        nrm = np.zeros((n, 3), np.float32) if normals else None
+        self.lib.orc_trace(self.h, _p(rays), n, _p(hits), _p(tests), _p(isects), _p(nrm), threads)
+        t = hits[:, 0].view(np.float32).copy()
+        out = (t, hits[:, 1].copy(), tests, isects)
+        return out + (nrm,) if normals else out
+
+    def render(self, cam, params, rgb32=True, rgb64=False, per_path=False, stream=None, threads=0):
+        """Returns (img32 or None, img64 or None, per-path or None, overflows)."""
+        import swpathtracer_amd._lib as L
+        p = L.RenderParams()
+        C.pointer(p)[0] = params
+        spp = _effective_spp(p.pixel_sampler, p.spp)
+        cnt = p.spp_count or (spp - p.spp_begin)
+        vw, vh = (p.viewport[2], p.viewport[3]) if p.viewport[2] else (cam.width, cam.height)
+        img32 = np.zeros((cam.height, cam.width, 3), np.float32) if rgb32 else None
+        img64 = np.zeros((cam.height, cam.width, 3), np.float64) if rgb64 else None
+        paths = np.zeros((vw * vh * cnt, 3), np.float32) if per_path else None
+        keep = None
+        if stream is not None:
+            keep = np.ascontiguousarray(stream, np.float32)
+            p.rng = L.SPT_RNG_STREAM
+            p.stream_stride = keep.shape[1]
+            p.d_stream = keep.ctypes.data
+        ov = C.c_uint64(0)
+        s = self.lib.orc_render(self.h, C.byref(cam), C.byref(p), _p(img32), _p(img64), _p(paths), C.byref(ov),
+                                threads)
+        assert s == 0, s
+        return img32, img64, paths, ov.value
+
+    def raycast(self, cam, kind, threads=0):
+        n = cam.width * cam.height
+        if kind == 0:
+            out = np.zeros((cam.height, cam.width, 3), np.float32)
+        elif kind == 1:
+            out = np.zeros((cam.height, cam.width), np.float32)
+        else:
+            out = np.zeros((cam.height, cam.width), np.uint32)
+        assert self.lib.orc_raycast(self.h, C.byref(cam), kind, _p(out), threads) == 0
+        return out
+
+
+def _effective_spp(sampler, spp):
+    if sampler == 2:
+        return spp
+    n = int(np.ceil(np.sqrt(spp)))
+    return n * n
+
+
+def camera_ray(cam, x, y):
+    out = np.zeros(8, np.float32)
+    assert load().orc_camera_ray(C.byref(cam), x, y, _p(out)) == 0
+    return out
+
+
+def pixel_sample(sampler, spp, sample, seed, path):
+    sx, sy = C.c_float(), C.c_float()
+    k = load().orc_pixel_sample(sampler, spp, sample, seed, path, C.byref(sx), C.byref(sy))
+    return sx.value, sy.value, k
+
+
+def rng_u(seed, path, k):
+    return load().orc_rng_u(seed, path, k)
+
+
+def material_sample(mat, wo, seed, path):
+    out = np.zeros(11, np.float32)
+    w = np.ascontiguousarray(wo, np.float32)
+    assert load().orc_material_sample(C.byref(mat), _p(w), seed, path, _p(out)) == 0
+    return out
+
+
+def layer_eval(layer, wo, wi):
+    out = np.zeros(4, np.float32)
+    a = np.ascontiguousarray(wo, np.float32)
+    b = np.ascontiguousarray(wi, np.float32)
+    assert load().orc_layer_eval(C.byref(layer), _p(a), _p(b), _p(out)) == 0
+    return out

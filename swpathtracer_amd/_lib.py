@@ -1,0 +1,143 @@
+"""ctypes declarations of the C ABI in include/spt.h (libspt.so).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C swpathtracer_amd/csrc``). Loading fails loudly when it is missing:
+there is no CPU fallback for the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspt.so")
+
+SPT_OK = 0
+SPT_ERR_INVALID = 1
+SPT_ERR_DEVICE = 2
+SPT_ERR_NOMEM = 3
+SPT_ERR_STATE = 4
+SPT_ERR_UNSUPPORTED = 5
+SPT_ERR_OVERFLOW = 6
+SPT_MISS = 0xFFFFFFFF
+
+SPT_BSDF_LAMBERTIAN = 1
+SPT_BSDF_COOK_TORRANCE = 2
+SPT_BSDF_SMOOTH_CONDUCTOR = 3
+SPT_BSDF_SMOOTH_DIELECTRIC = 4
+SPT_BSDF_GGX_CONDUCTOR = 5
+SPT_SAMPLER_IMPORTANCE = 0
+SPT_SAMPLER_UNIFORM = 1
+SPT_MAX_LAYERS = 4
+SPT_CAMERA_PINHOLE = 0
+SPT_CAMERA_ORTHOGRAPHIC = 1
+SPT_PIXEL_REGULAR = 0
+SPT_PIXEL_JITTERED = 1
+SPT_PIXEL_UNIFORM = 2
+SPT_RNG_COUNTER = 0
+SPT_RNG_STREAM = 1
+SPT_RENDER_FILM32 = 1 << 0
+SPT_RENDER_FILM64 = 1 << 1
+SPT_RENDER_PER_PATH = 1 << 2
+SPT_RENDER_NO_SORT = 1 << 3
+SPT_RENDER_GUIDED = 1 << 4
+SPT_RENDER_FETCH_STATS = 1 << 5
+SPT_RENDER_TIME_KERNELS = 1 << 6
+SPT_TRACE_CLOSEST = 0
+SPT_TRACE_COUNT = 1
+SPT_TRACE_UNPRUNED = 2
+SPT_TRACE_FETCH = 3
+SPT_RAYCAST_NORMAL = 0
+SPT_RAYCAST_DEPTH = 1
+SPT_RAYCAST_COUNT = 2
+
+F3 = C.c_float * 3
+
+
+class Layer(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("sampler", C.c_uint32), ("kd", F3), ("r0", F3),
+                ("roughness", C.c_float), ("ior", C.c_float), ("eta", F3), ("k", F3)]
+
+
+class Material(C.Structure):
+    _fields_ = [("n_layers", C.c_uint32), ("emission", F3), ("layers", Layer * SPT_MAX_LAYERS)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("pad", C.c_uint32),
+                ("position", F3), ("look_at", F3), ("up", F3), ("fov_deg", C.c_float),
+                ("min_x", C.c_float), ("max_x", C.c_float), ("min_y", C.c_float), ("max_y", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("pixel_sampler", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_int32),
+                ("rr_depth", C.c_int32), ("seed", C.c_uint64), ("spp_begin", C.c_uint32),
+                ("spp_count", C.c_uint32), ("viewport", C.c_uint32 * 4), ("background", F3),
+                ("flags", C.c_uint32), ("rng", C.c_uint32), ("stream_stride", C.c_uint32),
+                ("d_stream", C.c_void_p)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("n_prims", C.c_uint64), ("n_nodes", C.c_uint64), ("n_inner", C.c_uint64),
+                ("n_leaves", C.c_uint64), ("tree_height", C.c_uint64), ("n_references", C.c_uint64),
+                ("min_leaf", C.c_uint64), ("max_leaf", C.c_uint64), ("n_pair_nodes", C.c_uint64),
+                ("max_depth", C.c_uint64), ("root_min", F3), ("root_max", F3), ("build_ms", C.c_double)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("bounces", C.c_uint64),
+                ("stream_overflows", C.c_uint64), ("ms", C.c_double), ("extend_ms", C.c_double),
+                ("extend_launches", C.c_uint64), ("node_fetches", C.c_uint64), ("tri_fetches", C.c_uint64)]
+
+
+assert C.sizeof(Layer) == 64 and C.sizeof(Material) == 272
+
+# name -> (restype, argtypes); every entry point declared in include/spt.h
+PROTOTYPES = {
+    "spt_bvh_build": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "spt_bvh_get_info": (C.c_int32, [C.c_void_p, C.POINTER(BvhInfo)]),
+    "spt_bvh_export": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "spt_bvh_dump": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "spt_bvh_destroy": (None, [C.c_void_p]),
+    "spt_create": (C.c_int32, [C.c_int, C.POINTER(C.c_void_p)]),
+    "spt_destroy": (None, [C.c_void_p]),
+    "spt_last_error": (C.c_char_p, [C.c_void_p]),
+    "spt_sync": (C.c_int32, [C.c_void_p]),
+    "spt_stream": (C.c_void_p, [C.c_void_p]),
+    "spt_malloc": (C.c_int32, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "spt_free": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "spt_memcpy_h2d": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "spt_memcpy_d2h": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "spt_set_scene": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "spt_upload_scene": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                     C.c_void_p, C.c_void_p]),
+    "spt_set_materials": (C.c_int32, [C.c_void_p, C.POINTER(Material), C.c_uint32]),
+    "spt_trace": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p,
+                              C.c_void_p]),
+    "spt_camera_rays": (C.c_int32, [C.c_void_p, C.POINTER(Camera), C.c_void_p]),
+    "spt_raycast": (C.c_int32, [C.c_void_p, C.POINTER(Camera), C.c_uint32, C.c_void_p]),
+    "spt_render": (C.c_int32, [C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams), C.c_void_p,
+                               C.c_void_p, C.c_void_p]),
+    "spt_render_get_stats": (C.c_int32, [C.c_void_p, C.POINTER(RenderStats)]),
+    "spt_event_record": (C.c_int32, [C.c_void_p, C.c_int]),
+    "spt_event_elapsed": (C.c_int32, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libspt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "or `make -C swpathtracer_amd/csrc`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

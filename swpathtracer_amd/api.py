@@ -1,0 +1,380 @@
+"""Python face of the C ABI: tree build, device context, ray casting and path
+tracing. Mirrors the reference's host interfaces: ``Bvh`` stands for
+SBVH::build (acceleration/sbvh.cpp:12-37), ``Device.trace`` for a batch of
+AccelStructure::intersect calls (acceleration/accel_structure.h:29-30),
+``Device.raycast`` for RayCaster<T>::render (integrators/raycaster.h:61-99)
+and ``Device.render`` for PathTracer::render (integrators/pathtracer.cpp:59-147).
+Errors raise ``SptError`` (the reference throws std::runtime_error).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+class SptError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"spt error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
+    return None if a is None else a.ctypes.data
+
+
+def _f3(v) -> L.F3:
+    return L.F3(*[float(x) for x in v])
+
+
+# ----------------------------------------------------------------------------
+# materials / cameras
+# ----------------------------------------------------------------------------
+
+def lambertian(kd=(0.75, 0.75, 0.75), sampler: str = "importance") -> L.Layer:
+    """Lambertian{kd, bsdf_sampler} (lua_core/bsdfs.lua:11-27 defaults)."""
+    return L.Layer(type=L.SPT_BSDF_LAMBERTIAN, sampler=_sampler(sampler), kd=_f3(kd))
+
+
+def cook_torrance(roughness=0.2, r0=(0.99, 0.99, 0.99), sampler: str = "importance") -> L.Layer:
+    """CookTorrance{roughness, reflectance_at_normal_incidence} (bsdfs.lua:61-78)."""
+    return L.Layer(type=L.SPT_BSDF_COOK_TORRANCE, sampler=_sampler(sampler), roughness=roughness, r0=_f3(r0))
+
+
+def smooth_conductor(r0=(0.99, 0.99, 0.99), sampler: str = "importance") -> L.Layer:
+    """SmoothConductor{reflectance_at_normal_incidence} (bsdfs.lua:29-43)."""
+    return L.Layer(type=L.SPT_BSDF_SMOOTH_CONDUCTOR, sampler=_sampler(sampler), r0=_f3(r0))
+
+
+def smooth_dielectric(ior=1.5) -> L.Layer:
+    """SmoothDielectric{ior} (bsdfs.lua:45-59)."""
+    return L.Layer(type=L.SPT_BSDF_SMOOTH_DIELECTRIC, sampler=0, ior=ior)
+
+
+GOLD_ETA = (0.143119, 0.374957, 1.44248)  # Au, complex IOR at ~650/550/450 nm
+GOLD_K = (3.98316, 2.38572, 1.60322)
+
+
+def ggx_conductor(alpha=0.2, eta=GOLD_ETA, k=GOLD_K, sampler: str = "importance") -> L.Layer:
+    """GGX microfacet conductor with exact complex-IOR Fresnel (no reference counterpart)."""
+    return L.Layer(type=L.SPT_BSDF_GGX_CONDUCTOR, sampler=_sampler(sampler), roughness=alpha, eta=_f3(eta),
+                   k=_f3(k))
+
+
+def _sampler(name: str) -> int:
+    if name not in ("importance", "uniform"):
+        raise ValueError("bsdf_sampler must be 'importance' or 'uniform'")
+    return L.SPT_SAMPLER_IMPORTANCE if name == "importance" else L.SPT_SAMPLER_UNIFORM
+
+
+def material(layers: Sequence[L.Layer], emission=(0.0, 0.0, 0.0)) -> L.Material:
+    """Material{bsdf = LayeredBSDF{...}, emission = Emission{...}}; layers[0] is the top."""
+    if not 1 <= len(layers) <= L.SPT_MAX_LAYERS:
+        raise ValueError("1..4 layers")
+    m = L.Material(n_layers=len(layers), emission=_f3(emission))
+    for i, l in enumerate(layers):
+        m.layers[i] = l
+    return m
+
+
+def pinhole(width, height, position=(0, 0, 5), look_at=(0, 0, 0), up=(0, 1, 0), fov=55.0) -> L.Camera:
+    """Camera{type='pinhole'} (lua_core/cameras.lua defaults: position {0,0,5}, fov 55)."""
+    return L.Camera(type=L.SPT_CAMERA_PINHOLE, width=width, height=height, position=_f3(position),
+                    look_at=_f3(look_at), up=_f3(up), fov_deg=fov)
+
+
+def orthographic(width, height, min_x, max_x, min_y, max_y, position=(0, 0, 5), look_at=(0, 0, 0),
+                 up=(0, 1, 0)) -> L.Camera:
+    return L.Camera(type=L.SPT_CAMERA_ORTHOGRAPHIC, width=width, height=height, position=_f3(position),
+                    look_at=_f3(look_at), up=_f3(up), min_x=min_x, max_x=max_x, min_y=min_y, max_y=max_y)
+
+
+@dataclass
+class Integrator:
+    """Integrator{type='path-tracer'} settings plus the capped RR mode.
+
+    termination 'max-length': max_depth = path_length; 'russian-roulette':
+    rr_depth = path_length (integrators.lua:8-17 defaults: RR, 5). max_depth
+    may be set together with RR for a hard cap.
+    """
+    spp: int = 16
+    pixel_sampler: str = "jittered"
+    termination: str = "russian-roulette"
+    path_length: int = 5
+    max_depth: Optional[int] = None
+    seed: int = 0
+    background: tuple = (0.0, 0.0, 0.0)
+
+    def params(self) -> L.RenderParams:
+        p = L.RenderParams()
+        p.pixel_sampler = {"regular": L.SPT_PIXEL_REGULAR, "jittered": L.SPT_PIXEL_JITTERED,
+                           "uniform": L.SPT_PIXEL_UNIFORM}[self.pixel_sampler]
+        p.spp = self.spp
+        if self.termination == "max-length":
+            p.max_depth, p.rr_depth = self.path_length, -1
+        elif self.termination == "russian-roulette":
+            p.max_depth, p.rr_depth = -1, self.path_length
+        else:
+            raise ValueError("termination must be 'max-length' or 'russian-roulette'")
+        if self.max_depth is not None:
+            p.max_depth = self.max_depth
+        p.seed = self.seed
+        p.background = _f3(self.background)
+        return p
+
+    def effective_spp(self) -> int:
+        if self.pixel_sampler == "uniform":
+            return self.spp
+        n = int(np.ceil(np.sqrt(self.spp)))
+        return n * n
+
+
+# ----------------------------------------------------------------------------
+# tree
+# ----------------------------------------------------------------------------
+
+NODE_DTYPE = np.dtype([("lmin", "<f4", 3), ("lmax", "<f4", 3), ("rmin", "<f4", 3), ("rmax", "<f4", 3),
+                       ("lref", "<u4"), ("lcnt", "<u4"), ("rref", "<u4"), ("rcnt", "<u4")])
+TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("e1", "<f4", 3), ("e2", "<f4", 3), ("prim_id", "<u4"),
+                      ("pad", "<u4", 2)])
+assert NODE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48
+
+
+class Bvh:
+    """Host SAH build over triangles (n, 9) float32 in scene order."""
+
+    def __init__(self, verts: np.ndarray, n_threads: int = 8):
+        self.lib = L.load()
+        v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 9)
+        self.verts = v
+        h = C.c_void_p()
+        s = self.lib.spt_bvh_build(v.ctypes.data, v.shape[0], n_threads, C.byref(h))
+        if s != L.SPT_OK:
+            raise SptError(s, "spt_bvh_build failed")
+        self.h = h
+
+    @property
+    def info(self) -> dict:
+        inf = L.BvhInfo()
+        self.lib.spt_bvh_get_info(self.h, C.byref(inf))
+        return {f: (list(getattr(inf, f)) if f in ("root_min", "root_max") else getattr(inf, f))
+                for f, _ in L.BvhInfo._fields_}
+
+    def export(self):
+        inf = self.info
+        nodes = np.zeros(inf["n_pair_nodes"], NODE_DTYPE)
+        tris = np.zeros(inf["n_prims"], TRI_DTYPE)
+        leaf = np.zeros(inf["n_prims"], np.uint32)
+        box = np.zeros(6, np.float32)
+        s = self.lib.spt_bvh_export(self.h, _ptr(nodes) if len(nodes) else None, _ptr(tris), _ptr(leaf), _ptr(box))
+        if s != L.SPT_OK:
+            raise SptError(s, "spt_bvh_export failed")
+        return nodes, tris, leaf, box
+
+    def dump(self, path: str):
+        s = self.lib.spt_bvh_dump(self.h, path.encode())
+        if s != L.SPT_OK:
+            raise SptError(s, "spt_bvh_dump failed")
+
+    def close(self):
+        if self.h:
+            self.lib.spt_bvh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------------------
+# device
+# ----------------------------------------------------------------------------
+
+class DeviceArray:
+    """Device allocation made through the C ABI (spt_malloc / spt_free)."""
+
+    def __init__(self, dev: "Device", nbytes: int):
+        self.dev = dev
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        dev._check(dev.lib.spt_malloc(dev.h, max(self.nbytes, 16), C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.dev._check(self.dev.lib.spt_memcpy_h2d(self.dev.h, self.ptr, a.ctypes.data, a.nbytes))
+        self.dev.sync()
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        self.dev._check(self.dev.lib.spt_memcpy_d2h(self.dev.h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def zero(self):
+        self.upload(np.zeros(self.nbytes, np.uint8))
+
+    def free(self):
+        if self.ptr:
+            self.dev.lib.spt_free(self.dev.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Device:
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        s = self.lib.spt_create(device, C.byref(h))
+        if s != L.SPT_OK:
+            raise SptError(s, f"spt_create({device}) failed (no HIP device?)")
+        self.h = h
+
+    def _check(self, s: int):
+        if s != L.SPT_OK:
+            raise SptError(s, self.lib.spt_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.spt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return self.lib.spt_stream(self.h)
+
+    def sync(self):
+        self._check(self.lib.spt_sync(self.h))
+
+    def event_record(self, slot: int):
+        self._check(self.lib.spt_event_record(self.h, slot))
+
+    def event_elapsed(self, a: int, b: int) -> float:
+        ms = C.c_float()
+        self._check(self.lib.spt_event_elapsed(self.h, a, b, C.byref(ms)))
+        return ms.value
+
+    def alloc(self, nbytes: int) -> DeviceArray:
+        return DeviceArray(self, nbytes)
+
+    def to_device(self, a: np.ndarray) -> DeviceArray:
+        d = DeviceArray(self, a.nbytes)
+        d.upload(a)
+        return d
+
+    # scene -------------------------------------------------------------
+    def set_scene(self, bvh: Bvh, normals: Optional[np.ndarray] = None, mat_ids: Optional[np.ndarray] = None):
+        n = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+        m = None if mat_ids is None else np.ascontiguousarray(mat_ids, np.uint32)
+        self._check(self.lib.spt_set_scene(self.h, bvh.h, bvh.verts.ctypes.data, _ptr(n), _ptr(m)))
+
+    def set_materials(self, mats: Sequence[L.Material]):
+        arr = (L.Material * len(mats))(*mats)
+        self._check(self.lib.spt_set_materials(self.h, arr, len(mats)))
+
+    # ray casting -------------------------------------------------------
+    def trace(self, d_rays: int, n: int, d_hits: int, mode: int = L.SPT_TRACE_CLOSEST,
+              d_tests: Optional[int] = None, d_isects: Optional[int] = None):
+        self._check(self.lib.spt_trace(self.h, d_rays, n, d_hits, mode, d_tests, d_isects))
+
+    def trace_host(self, rays: np.ndarray, mode: int = L.SPT_TRACE_CLOSEST):
+        """rays (n, 8) float32 -> (t, prim_id[, tests, isects]) as numpy."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        dr = self.to_device(rays)
+        dh = self.alloc(8 * n)
+        counted = mode in (L.SPT_TRACE_COUNT, L.SPT_TRACE_FETCH)
+        dt = self.alloc(4 * n) if counted else None
+        di = self.alloc(4 * n) if counted else None
+        self.trace(dr.ptr, n, dh.ptr, mode, dt.ptr if dt else None, di.ptr if di else None)
+        self.sync()
+        h = dh.download(np.uint32, (n, 2))
+        t = h[:, 0].view(np.float32).copy()
+        prim = h[:, 1].copy()
+        if counted:
+            return t, prim, dt.download(np.uint32, n), di.download(np.uint32, n)
+        return t, prim
+
+    def camera_rays(self, cam: L.Camera) -> np.ndarray:
+        """Pixel-centre primary rays (h*w, 8) float32, computed on the device."""
+        n = cam.width * cam.height
+        d = self.alloc(32 * n)
+        self._check(self.lib.spt_camera_rays(self.h, C.byref(cam), d.ptr))
+        self.sync()
+        out = d.download(np.float32, (n, 8))
+        d.free()
+        return out
+
+    def raycast(self, cam: L.Camera, kind: int) -> np.ndarray:
+        n = cam.width * cam.height
+        comps = 3 if kind == L.SPT_RAYCAST_NORMAL else 1
+        d = self.alloc(4 * n * comps)
+        self._check(self.lib.spt_raycast(self.h, C.byref(cam), kind, d.ptr))
+        self.sync()
+        dt = np.uint32 if kind == L.SPT_RAYCAST_COUNT else np.float32
+        out = d.download(dt, (cam.height, cam.width, comps) if comps == 3 else (cam.height, cam.width))
+        d.free()
+        return out
+
+    # path tracing ------------------------------------------------------
+    def render(self, cam: L.Camera, params: L.RenderParams, d_rgb32=None, d_rgb64=None, d_path=None):
+        self._check(self.lib.spt_render(self.h, C.byref(cam), C.byref(params), d_rgb32, d_rgb64, d_path))
+
+    def render_stats(self) -> dict:
+        st = L.RenderStats()
+        self._check(self.lib.spt_render_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in L.RenderStats._fields_}
+
+    def render_image(self, cam: L.Camera, integ: Integrator, film64: bool = False, per_path: bool = False,
+                     spp_begin: int = 0, spp_count: int = 0, viewport=None, stream: Optional[np.ndarray] = None):
+        """Renders and returns (image (h, w, 3) as the reference's pixel values,
+        per-path radiance or None). Host convenience for tests and tools."""
+        p = integ.params()
+        spp = integ.effective_spp()
+        cnt = spp_count or (spp - spp_begin)
+        p.spp_begin, p.spp_count = spp_begin, cnt
+        if viewport is not None:
+            p.viewport = (C.c_uint32 * 4)(*viewport)
+            vw, vh = viewport[2], viewport[3]
+        else:
+            vw, vh = cam.width, cam.height
+        npix = cam.width * cam.height
+        film = self.alloc(npix * 3 * (8 if film64 else 4))
+        film.zero()
+        p.flags = L.SPT_RENDER_FILM64 if film64 else L.SPT_RENDER_FILM32
+        dpath = None
+        if per_path:
+            dpath = self.alloc(vw * vh * cnt * 12)
+            p.flags |= L.SPT_RENDER_PER_PATH
+        dstream = None
+        if stream is not None:
+            stream = np.ascontiguousarray(stream, np.float32)
+            dstream = self.to_device(stream)
+            p.rng = L.SPT_RNG_STREAM
+            p.stream_stride = stream.shape[1]
+            p.d_stream = dstream.ptr
+        self.render(cam, p, None if film64 else film.ptr, film.ptr if film64 else None,
+                    dpath.ptr if dpath else None)
+        self.sync()
+        img = film.download(np.float64 if film64 else np.float32, (cam.height, cam.width, 3))
+        img = img / (np.float64(cnt) if film64 else np.float32(cnt))
+        paths = dpath.download(np.float32, (vw * vh * cnt, 3)) if dpath else None
+        return img, paths

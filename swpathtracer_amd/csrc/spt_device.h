@@ -1,0 +1,260 @@
+// spt_device.h - device-side scene layout, context and closest-hit traversal.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "spt.h"
+#include "spt_internal.h"
+#include "spt_math.h"
+
+namespace spt {
+
+constexpr int kBlock = 256;       // threads per block of the ray kernels (4 wave64)
+constexpr int kLdsStack = 16;     // traversal stack entries per lane kept in LDS
+constexpr uint32_t kNoHit = SPT_MISS;
+
+// Scene as the kernels see it.
+struct DevScene {
+    const float4* nodes;    // n_nodes x 4 float4 (spt_node64)
+    const float4* tris;     // n_tris x 3 float4 (spt_tri48; .pad[0] = material id)
+    const float4* shading;  // n_tris x 3 float4: n0 n1 n2, material, smooth flag, pad
+    const spt_material* mats;
+    float rmin[3], rmax[3];
+    uint32_t root_leaf;  // > 0: the root is a leaf holding this many triangles
+    uint32_t n_tris;
+    uint32_t n_nodes;
+    uint32_t n_mats;
+    uint32_t stack_cap;  // traversal stack entries a lane may need (inner-node depth + 1)
+};
+
+// Closest-hit result of one ray.
+struct Hit {
+    float t;
+    uint32_t idx;   // triangle index in leaf order (kNoHit: miss)
+    uint32_t prim;  // primitive id in scene order
+    uint32_t mat;
+    float u, v;
+};
+
+// Traversal stack: the first kLdsStack entries of a lane live in LDS
+// (layout [entry][lane], conflict-free), deeper entries spill to a global
+// buffer laid out [entry - kLdsStack][thread] so a wave's accesses coalesce.
+struct Stack {
+    uint32_t* lds;      // &lds[threadIdx.x], stride kBlock
+    uint32_t* spill;    // &spill[global thread], stride spill_stride
+    uint32_t spill_stride;
+    uint32_t cap;       // total entries available
+    __device__ void put(int i, uint32_t v) {
+        if (i < kLdsStack)
+            lds[i * kBlock] = v;
+        else
+            spill[(size_t)(i - kLdsStack) * spill_stride] = v;
+    }
+    __device__ uint32_t get(int i) const {
+        return (i < kLdsStack) ? lds[i * kBlock] : spill[(size_t)(i - kLdsStack) * spill_stride];
+    }
+};
+
+// Leaf: tests triangles [first, first+cnt) against the ray. A candidate is
+// accepted exactly when the reference accepts it (sbvh.cpp:494-496: t > 0,
+// t < best, hit point inside the leaf box +-1e-3); among equal t the smaller
+// leaf-order index wins, which is the reference's depth-first left-first
+// first-come rule (strict '<').
+template <bool COUNT, bool FETCH>
+__device__ inline void leaf_test(const DevScene& sc, V3 o, V3 d, uint32_t first, uint32_t cnt, V3 bmn, V3 bmx,
+                                 Hit& h, uint32_t& tests, uint32_t& isects) {
+    if (FETCH) isects += cnt;  // triangle records fetched
+    for (uint32_t k = first; k < first + cnt; ++k) {
+        const float4* tp = sc.tris + 3 * (size_t)k;
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        if (COUNT) ++tests;
+        float t, u, v;
+        if (tri_test(o, d, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, t, u, v)) {
+            if (COUNT) ++isects;
+            if ((t > 0.0f) && ((t < h.t) || ((t == h.t) && (k < h.idx)))) {
+                V3 p = o + d * t;
+                if (point_inside(p, bmn, bmx)) {
+                    h.t = t;
+                    h.idx = k;
+                    h.prim = __float_as_uint(c.y);
+                    h.mat = __float_as_uint(c.z);
+                    h.u = u;
+                    h.v = v;
+                }
+            }
+        }
+    }
+}
+
+// Closest hit of one ray (acceleration/sbvh.cpp:423-515 semantics).
+//   PRUNE = true : near-first order, children skipped when their slab interval
+//                  starts after the best hit or ends behind the origin. Child
+//                  boxes are padded by 1e-3 around their triangles, so a
+//                  skipped box cannot hold an accepted candidate; the result
+//                  equals the reference's.
+//   PRUNE = false: visits exactly the nodes the reference visits (every box
+//                  the ray's line meets) and, with COUNT, reproduces its
+//                  num_intersection_tests / num_intersections.
+//   FETCH = true : (without COUNT) tests/isects receive the number of 64-B
+//                  node records and 48-B triangle records this ray fetched.
+template <bool PRUNE, bool COUNT, bool FETCH = false>
+__device__ inline void trace_ray(const DevScene& sc, V3 o, V3 d, Stack& st, Hit& h, uint32_t& tests,
+                                 uint32_t& isects, bool& overflow) {
+    h.t = __builtin_inff();
+    h.idx = kNoHit;
+    h.prim = kNoHit;
+    h.mat = 0;
+    h.u = h.v = 0.0f;
+    tests = 0;
+    isects = 0;
+    const V3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    float tn, tf;
+    const V3 rmn{sc.rmin[0], sc.rmin[1], sc.rmin[2]}, rmx{sc.rmax[0], sc.rmax[1], sc.rmax[2]};
+    if (COUNT) tests = 1;
+    if (!slab(o, inv, rmn, rmx, tn, tf)) return;
+    if (COUNT) isects = 1;
+    if (sc.root_leaf) {
+        leaf_test<COUNT, FETCH>(sc, o, d, 0, sc.root_leaf, rmn, rmx, h, tests, isects);
+        return;
+    }
+    uint32_t node = 0;
+    int sp = 0;
+    while (true) {
+        const float4* np = sc.nodes + 4 * (size_t)node;
+        const float4 a = np[0], b = np[1], c = np[2];
+        const uint4 r = reinterpret_cast<const uint4*>(np)[3];
+        if (FETCH) ++tests;
+        const V3 lmn{a.x, a.y, a.z}, lmx{a.w, b.x, b.y}, rmn2{b.z, b.w, c.x}, rmx2{c.y, c.z, c.w};
+        float tnl = 0.0f, tfl = 0.0f, tnr = 0.0f, tfr = 0.0f;
+        bool hl = slab(o, inv, lmn, lmx, tnl, tfl);
+        bool hr = slab(o, inv, rmn2, rmx2, tnr, tfr);
+        if (COUNT) {
+            tests += 2;
+            isects += (uint32_t)hl + (uint32_t)hr;
+        }
+        if (PRUNE) {
+            hl = hl && !(tnl > h.t) && !(tfl < 0.0f);
+            hr = hr && !(tnr > h.t) && !(tfr < 0.0f);
+        }
+        if (hl && r.y) {
+            leaf_test<COUNT, FETCH>(sc, o, d, r.x, r.y, lmn, lmx, h, tests, isects);
+            hl = false;
+        }
+        if (hr && r.w) {
+            leaf_test<COUNT, FETCH>(sc, o, d, r.z, r.w, rmn2, rmx2, h, tests, isects);
+            hr = false;
+        }
+        if (hl && hr) {
+            const bool rfirst = PRUNE && (tnr < tnl);
+            if (sp >= (int)st.cap) {
+                overflow = true;
+                return;
+            }
+            st.put(sp++, rfirst ? r.x : r.z);
+            node = rfirst ? r.z : r.x;
+        } else if (hl) {
+            node = r.x;
+        } else if (hr) {
+            node = r.z;
+        } else {
+            if (sp == 0) break;
+            node = st.get(--sp);
+        }
+    }
+}
+
+// Camera ray through image point (x, y): PinholeCamera::getRay
+// (cameras/pinhole_camera.cpp:14-18), OrthographicCamera::getRay
+// (cameras/orthographic_camera.cpp:21-25).
+__device__ inline void camera_ray(const DevCamera& c, float x, float y, V3& o, V3& d) {
+    if (c.type == SPT_CAMERA_PINHOLE) {
+        V3 ld = normalize(c.ulc + V3{x, -y, 0.0f});
+        o = c.position;
+        d = to_world(c.frame, ld);
+    } else {
+        V3 lo = c.ulc + V3{x * c.pdx, y * -c.pdy, 0.0f};
+        o = to_world(c.frame, lo) + c.position;
+        d = c.ortho_dir;
+    }
+}
+
+// Shading normal of a hit: SmoothTriangle::getNormal
+// (primitives/smooth_triangle.h:11-13) or the stored flat normal
+// (primitives/triangle.cpp:5-6).
+__device__ inline V3 shading_normal(const DevScene& sc, const Hit& h) {
+    const float4* sp = sc.shading + 3 * (size_t)h.idx;
+    const float4 a = sp[0], b = sp[1], c = sp[2];
+    const V3 n0{a.x, a.y, a.z};
+    if (__float_as_uint(c.z) == 0u) return n0;
+    const V3 n1{a.w, b.x, b.y}, n2{b.z, b.w, c.x};
+    return normalize(n0 * (1.0f - h.u - h.v) + n1 * h.u + n2 * h.v);
+}
+
+}  // namespace spt
+
+// The context: one device, one stream, the uploaded scene and scratch buffers.
+struct spt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cus = 256;
+    // scene
+    float4* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    float4* d_shading = nullptr;
+    spt_material* d_mats = nullptr;
+    uint32_t n_nodes = 0, n_tris = 0, n_mats = 0;
+    float root_box[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t root_leaf = 0;
+    uint32_t stack_cap = 0;
+    uint32_t max_mat = 0;  // largest material id referenced by a triangle
+    bool has_scene = false;
+    std::vector<spt_material> h_mats;
+    // traversal stack spill buffer
+    uint32_t* d_spill = nullptr;
+    size_t spill_words = 0;
+    // status word written by kernels (1 = stack overflow), then two 64-bit
+    // fetch counters (nodes, triangles) at byte offset 16
+    uint32_t* d_flags = nullptr;
+    // HIP events: [0..1] bracket extend launches, [2..9] user slots (spt_event_*)
+    hipEvent_t ev[10] = {};
+    // path-tracing state (grown on demand)
+    void* d_paths = nullptr;
+    size_t paths_bytes = 0;
+    spt_render_stats stats{};
+
+    spt::DevScene scene() const {
+        spt::DevScene s;
+        s.nodes = d_nodes;
+        s.tris = d_tris;
+        s.shading = d_shading;
+        s.mats = d_mats;
+        for (int i = 0; i < 3; ++i) {
+            s.rmin[i] = root_box[i];
+            s.rmax[i] = root_box[3 + i];
+        }
+        s.root_leaf = root_leaf;
+        s.n_tris = n_tris;
+        s.n_nodes = n_nodes;
+        s.n_mats = n_mats;
+        s.stack_cap = stack_cap;
+        return s;
+    }
+};
+
+namespace spt {
+spt_status set_error(spt_ctx* ctx, spt_status code, const std::string& msg);
+spt_status check_hip(spt_ctx* ctx, hipError_t e, const char* what);
+// grows ctx->d_spill to hold `words` entries; returns false on failure
+spt_status ensure_spill(spt_ctx* ctx, size_t words);
+// launch helpers implemented by the kernel translation units
+spt_status launch_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits, uint32_t mode,
+                        uint32_t* d_tests, uint32_t* d_isects);
+spt_status launch_raycast(spt_ctx* ctx, const spt_camera* cam, uint32_t kind, void* d_out);
+spt_status launch_camera_rays(spt_ctx* ctx, const spt_camera* cam, spt_ray32* d_rays);
+spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* p, float* d_rgb32,
+                         double* d_rgb64, float* d_path_rgb);
+}  // namespace spt

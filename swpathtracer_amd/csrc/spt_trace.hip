@@ -1,0 +1,145 @@
+// spt_trace.hip - closest-hit ray casting kernels (batched
+// AccelStructure::intersect, acceleration/accel_structure.h:29-30) and the
+// three ray casters (integrators/raycaster.h:61-99).
+#include "spt_device.h"
+
+namespace spt {
+
+namespace {
+
+template <bool PRUNE, bool COUNT, bool FETCH>
+__global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, const float4* __restrict__ rays, uint64_t n,
+                                                  uint2* __restrict__ hits, uint32_t* __restrict__ tests_out,
+                                                  uint32_t* __restrict__ isects_out, uint32_t* __restrict__ spill,
+                                                  uint32_t* __restrict__ flags) {
+    __shared__ uint32_t lds[kLdsStack * kBlock];
+    const uint32_t gthreads = gridDim.x * kBlock;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    Stack st{lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap};
+    bool overflow = false;
+    for (uint64_t i = gtid; i < n; i += gthreads) {
+        const float4 ro = rays[2 * i], rd = rays[2 * i + 1];
+        Hit h;
+        uint32_t tests, isects;
+        trace_ray<PRUNE, COUNT, FETCH>(sc, V3{ro.x, ro.y, ro.z}, V3{rd.x, rd.y, rd.z}, st, h, tests, isects,
+                                       overflow);
+        hits[i] = make_uint2(__float_as_uint(h.t), h.prim);
+        if (COUNT || FETCH) {
+            if (tests_out) tests_out[i] = tests;
+            if (isects_out) isects_out[i] = isects;
+        }
+    }
+    if (overflow) atomicOr(flags, 1u);
+}
+
+// One ray per pixel centre (RegularPixelSampler(1) gives (0.5, 0.5),
+// regular_pixel_sampler.cpp:3-14); value per normal_raycaster.cpp:19-31,
+// depth_raycaster.cpp:36-48, int_test_count_raycaster.cpp:75-85.
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_raycast(DevScene sc, DevCamera cam, void* __restrict__ out,
+                                                    uint32_t* __restrict__ spill, uint32_t* __restrict__ flags) {
+    __shared__ uint32_t lds[kLdsStack * kBlock];
+    const uint32_t gthreads = gridDim.x * kBlock;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    Stack st{lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap};
+    bool overflow = false;
+    const uint64_t n = (uint64_t)cam.width * cam.height;
+    for (uint64_t i = gtid; i < n; i += gthreads) {
+        const uint32_t x = (uint32_t)(i % cam.width), y = (uint32_t)(i / cam.width);
+        V3 o, d;
+        camera_ray(cam, x + 0.5f, y + 0.5f, o, d);
+        Hit h;
+        uint32_t tests, isects;
+        if (KIND == SPT_RAYCAST_COUNT) {
+            trace_ray<false, true>(sc, o, d, st, h, tests, isects, overflow);
+            reinterpret_cast<uint32_t*>(out)[i] = tests;
+        } else {
+            trace_ray<true, false>(sc, o, d, st, h, tests, isects, overflow);
+            if (KIND == SPT_RAYCAST_DEPTH) {
+                reinterpret_cast<float*>(out)[i] = (h.idx != kNoHit) ? h.t : -1.0f;
+            } else {
+                V3 nrm = (h.idx != kNoHit) ? shading_normal(sc, h) : V3{0.0f, 0.0f, 0.0f};
+                float* o3 = reinterpret_cast<float*>(out) + 3 * i;
+                o3[0] = nrm.x;
+                o3[1] = nrm.y;
+                o3[2] = nrm.z;
+            }
+        }
+    }
+    if (overflow) atomicOr(flags, 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_camera_rays(DevCamera cam, float4* __restrict__ rays) {
+    const uint64_t n = (uint64_t)cam.width * cam.height;
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = (uint32_t)(i % cam.width), y = (uint32_t)(i / cam.width);
+    V3 o, d;
+    camera_ray(cam, x + 0.5f, y + 0.5f, o, d);
+    rays[2 * i] = make_float4(o.x, o.y, o.z, 0.0f);
+    rays[2 * i + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+}
+
+uint32_t grid_for(const spt_ctx* ctx, uint64_t n) {
+    uint64_t want = (n + kBlock - 1) / kBlock;
+    uint64_t cap = (uint64_t)ctx->num_cus * 16;  // resident-size grid, stride loop beyond
+    if (want > cap) want = cap;
+    if (want == 0) want = 1;
+    return (uint32_t)want;
+}
+
+}  // namespace
+
+spt_status launch_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits, uint32_t mode,
+                        uint32_t* d_tests, uint32_t* d_isects) {
+    if (n == 0) return SPT_OK;
+    const uint32_t grid = grid_for(ctx, n);
+    spt_status s = ensure_spill(ctx, (size_t)grid * kBlock * (ctx->stack_cap > kLdsStack ? ctx->stack_cap - kLdsStack : 1));
+    if (s != SPT_OK) return s;
+    DevScene sc = ctx->scene();
+    const float4* rays = reinterpret_cast<const float4*>(d_rays);
+    uint2* hits = reinterpret_cast<uint2*>(d_hits);
+    if (mode == SPT_TRACE_CLOSEST)
+        hipLaunchKernelGGL((k_trace<true, false, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, n,
+                           hits, nullptr, nullptr, ctx->d_spill, ctx->d_flags);
+    else if (mode == SPT_TRACE_COUNT)
+        hipLaunchKernelGGL((k_trace<false, true, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, n,
+                           hits, d_tests, d_isects, ctx->d_spill, ctx->d_flags);
+    else if (mode == SPT_TRACE_FETCH)
+        hipLaunchKernelGGL((k_trace<true, false, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, n,
+                           hits, d_tests, d_isects, ctx->d_spill, ctx->d_flags);
+    else
+        hipLaunchKernelGGL((k_trace<false, false, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, n,
+                           hits, nullptr, nullptr, ctx->d_spill, ctx->d_flags);
+    return check_hip(ctx, hipGetLastError(), "trace launch");
+}
+
+spt_status launch_camera_rays(spt_ctx* ctx, const spt_camera* cam, spt_ray32* d_rays) {
+    const uint64_t n = (uint64_t)cam->width * cam->height;
+    const uint32_t grid = (uint32_t)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_camera_rays, dim3(grid), dim3(kBlock), 0, ctx->stream, make_camera(*cam),
+                       reinterpret_cast<float4*>(d_rays));
+    return check_hip(ctx, hipGetLastError(), "camera rays launch");
+}
+
+spt_status launch_raycast(spt_ctx* ctx, const spt_camera* cam, uint32_t kind, void* d_out) {
+    const uint64_t n = (uint64_t)cam->width * cam->height;
+    if (n == 0) return SPT_OK;
+    const uint32_t grid = grid_for(ctx, n);
+    spt_status s = ensure_spill(ctx, (size_t)grid * kBlock * (ctx->stack_cap > kLdsStack ? ctx->stack_cap - kLdsStack : 1));
+    if (s != SPT_OK) return s;
+    DevScene sc = ctx->scene();
+    DevCamera dc = make_camera(*cam);
+    if (kind == SPT_RAYCAST_NORMAL)
+        hipLaunchKernelGGL((k_raycast<SPT_RAYCAST_NORMAL>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, dc, d_out,
+                           ctx->d_spill, ctx->d_flags);
+    else if (kind == SPT_RAYCAST_DEPTH)
+        hipLaunchKernelGGL((k_raycast<SPT_RAYCAST_DEPTH>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, dc, d_out,
+                           ctx->d_spill, ctx->d_flags);
+    else
+        hipLaunchKernelGGL((k_raycast<SPT_RAYCAST_COUNT>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, dc, d_out,
+                           ctx->d_spill, ctx->d_flags);
+    return check_hip(ctx, hipGetLastError(), "raycast launch");
+}
+
+}  // namespace spt
