@@ -213,6 +213,7 @@ spt_status spt_memcpy_d2h(spt_ctx* ctx, void* dst, const void* d_src, size_t byt
 spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_nodes, const spt_tri48* tris,
                             uint32_t n_tris, const float* root_box, const float* shading) {
     if (!ctx || !tris || n_tris == 0 || !root_box || (n_nodes && !nodes)) return SPT_ERR_INVALID;
+    if (n_nodes >= (1u << 30)) return set_error(ctx, SPT_ERR_UNSUPPORTED, "more than 2^30 nodes");
     // every child reference must stay inside the arrays the kernels read
     for (uint32_t i = 0; i < n_nodes; ++i) {
         const spt_node64& nd = nodes[i];

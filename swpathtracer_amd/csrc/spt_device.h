@@ -166,6 +166,149 @@ __device__ inline void trace_ray(const DevScene& sc, V3 o, V3 d, Stack& st, Hit&
     }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent closest-hit traversal (the throughput path).
+//
+// Each wave loops until the ray pool is empty: idle lanes take the next rays
+// with one wave-aggregated atomic, then every lane runs an inner-node phase
+// (descend near-first, push the far child) followed by a leaf phase (test the
+// triangles of the leaf it stopped at, pop the next entry). Leaves are stack
+// entries too, encoded as kLeafBit | node << 1 | side, so a lane that reaches
+// a leaf waits in the leaf phase for the wave instead of branching alone; the
+// leaf's box (needed for the reference's hit-point-inside-leaf test) is reread
+// from its parent node, which is cache-resident by then.
+// Same candidate acceptance and tie rule as trace_ray<true, false>.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kDone = 0xFFFFFFFFu;
+
+template <bool FETCH>
+struct PLane {
+    V3 o, d, inv;
+    Hit h;
+    uint32_t cur;   // node index, leaf entry, or kDone
+    int sp;
+    uint32_t ray;   // index in the batch (kDone: idle)
+    uint32_t nodes, tris;  // FETCH counters
+};
+
+// GetRay(i, o, d) loads ray i; PutHit(i, h, nodes, tris) stores its result.
+template <bool FETCH, typename GetRay, typename PutHit>
+__device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t* ray_counter, Stack& st,
+                                        bool& overflow, GetRay get_ray, PutHit put_hit) {
+    const V3 rmn{sc.rmin[0], sc.rmin[1], sc.rmin[2]}, rmx{sc.rmax[0], sc.rmax[1], sc.rmax[2]};
+    const uint64_t lt = __lanemask_lt();
+    PLane<FETCH> L;
+    L.ray = kDone;
+    L.cur = kDone;
+    L.sp = 0;
+    bool pool_empty = false;
+    while (true) {
+        // ---- refill idle lanes
+        const uint64_t idle = __ballot(L.ray == kDone);
+        if (!pool_empty && __popcll(idle) >= 16) {
+            uint32_t base = 0;
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            if (__lane_id() == (unsigned)leader) base = atomicAdd(ray_counter, (uint32_t)__popcll(idle));
+            base = __shfl(base, leader);
+            if (base + __popcll(idle) >= n) pool_empty = true;
+            if (L.ray == kDone) {
+                const uint32_t i = base + (uint32_t)__popcll(idle & lt);
+                if (i < n) {
+                    L.ray = i;
+                    get_ray(i, L.o, L.d);
+                    L.inv = V3{1.0f / L.d.x, 1.0f / L.d.y, 1.0f / L.d.z};
+                    L.h.t = __builtin_inff();
+                    L.h.idx = kNoHit;
+                    L.h.prim = kNoHit;
+                    L.h.mat = 0;
+                    L.h.u = L.h.v = 0.0f;
+                    L.sp = 0;
+                    if (FETCH) L.nodes = L.tris = 0;
+                    float tn, tf;
+                    if (!slab(L.o, L.inv, rmn, rmx, tn, tf))
+                        L.cur = kDone;
+                    else
+                        L.cur = sc.root_leaf ? kLeafBit : 0u;
+                }
+            }
+        }
+        if (__ballot(L.ray != kDone) == 0) {
+            if (pool_empty) break;
+            continue;
+        }
+        // ---- inner-node phase
+        while (L.ray != kDone && L.cur != kDone && !(L.cur & kLeafBit)) {
+            const uint32_t node = L.cur;
+            const float4* np = sc.nodes + 4 * (size_t)node;
+            const float4 a = np[0], b = np[1], c = np[2];
+            const uint4 r = reinterpret_cast<const uint4*>(np)[3];
+            if (FETCH) ++L.nodes;
+            float tnl = 0.0f, tfl = 0.0f, tnr = 0.0f, tfr = 0.0f;
+            const bool hl = slab(L.o, L.inv, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, tnl, tfl) && !(tnl > L.h.t) &&
+                            !(tfl < 0.0f);
+            const bool hr = slab(L.o, L.inv, V3{b.z, b.w, c.x}, V3{c.y, c.z, c.w}, tnr, tfr) && !(tnr > L.h.t) &&
+                            !(tfr < 0.0f);
+            const uint32_t el = r.y ? (kLeafBit | (node << 1)) : r.x;
+            const uint32_t er = r.w ? (kLeafBit | (node << 1) | 1u) : r.z;
+            if (hl && hr) {
+                const bool rfirst = tnr < tnl;
+                if (L.sp >= (int)st.cap) {
+                    overflow = true;
+                    L.cur = kDone;
+                    L.sp = 0;
+                    break;
+                }
+                st.put(L.sp++, rfirst ? el : er);
+                L.cur = rfirst ? er : el;
+            } else if (hl) {
+                L.cur = el;
+            } else if (hr) {
+                L.cur = er;
+            } else {
+                L.cur = (L.sp > 0) ? st.get(--L.sp) : kDone;
+            }
+        }
+        // ---- leaf phase
+        while (L.ray != kDone && L.cur != kDone && (L.cur & kLeafBit)) {
+            uint32_t first, cnt;
+            V3 bmn, bmx;
+            if (sc.root_leaf) {
+                first = 0;
+                cnt = sc.root_leaf;
+                bmn = rmn;
+                bmx = rmx;
+            } else {
+                const uint32_t node = (L.cur & ~kLeafBit) >> 1;
+                const float4* np = sc.nodes + 4 * (size_t)node;
+                const uint4 r = reinterpret_cast<const uint4*>(np)[3];
+                if (L.cur & 1u) {
+                    const float4 b = np[1], c = np[2];
+                    bmn = V3{b.z, b.w, c.x};
+                    bmx = V3{c.y, c.z, c.w};
+                    first = r.z;
+                    cnt = r.w;
+                } else {
+                    const float4 a = np[0], b = np[1];
+                    bmn = V3{a.x, a.y, a.z};
+                    bmx = V3{a.w, b.x, b.y};
+                    first = r.x;
+                    cnt = r.y;
+                }
+            }
+            uint32_t dummy_t = 0, dummy_i = 0;
+            leaf_test<false, false>(sc, L.o, L.d, first, cnt, bmn, bmx, L.h, dummy_t, dummy_i);
+            if (FETCH) L.tris += cnt;
+            L.cur = (L.sp > 0) ? st.get(--L.sp) : kDone;
+        }
+        // ---- finished rays
+        if (L.ray != kDone && L.cur == kDone) {
+            put_hit(L.ray, L.h, FETCH ? L.nodes : 0u, FETCH ? L.tris : 0u);
+            L.ray = kDone;
+        }
+    }
+}
+
 // Camera ray through image point (x, y): PinholeCamera::getRay
 // (cameras/pinhole_camera.cpp:14-18), OrthographicCamera::getRay
 // (cameras/orthographic_camera.cpp:21-25).
@@ -191,6 +334,27 @@ __device__ inline V3 shading_normal(const DevScene& sc, const Hit& h) {
     if (__float_as_uint(c.z) == 0u) return n0;
     const V3 n1{a.w, b.x, b.y}, n2{b.z, b.w, c.x};
     return normalize(n0 * (1.0f - h.u - h.v) + n1 * h.u + n2 * h.v);
+}
+
+// Wave-aggregated counter increment: lanes of the wave that carry the same
+// key share one global atomicAdd (one per distinct key per wave instead of one
+// per lane); returns this lane's slot (old value + rank among equal keys).
+// Must be called by every active lane of the wave together.
+__device__ inline uint32_t wave_add(uint32_t* counters, uint32_t key) {
+    uint64_t todo = __ballot(1);
+    const uint64_t lt = __lanemask_lt();
+    uint32_t slot = 0;
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lkey = __shfl(key, leader);
+        const uint64_t group = __ballot(key == lkey) & todo;
+        uint32_t base = 0;
+        if (__lane_id() == (unsigned)leader) base = atomicAdd(&counters[lkey], (uint32_t)__popcll(group));
+        base = __shfl(base, leader);
+        if (key == lkey) slot = base + (uint32_t)__popcll(group & lt);
+        todo &= ~group;
+    }
+    return slot;
 }
 
 }  // namespace spt

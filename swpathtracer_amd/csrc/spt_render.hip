@@ -156,8 +156,53 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, Paths p, const 
         p.hit[s] = make_float4(h.t, __uint_as_float(h.idx), h.u, h.v);
         const uint32_t key = (h.idx == kNoHit) ? a.sc.n_mats : h.mat;
         p.mat[s] = key;
-        if (sort_keys) atomicAdd(&p.counters[2 + key], 1u);
+        if (sort_keys) (void)wave_add(p.counters + 2, key);
     }
+    if (overflow) atomicOr(flags, 1u);
+    if (FETCH) {
+        atomicAdd(&fetched[0], my_nodes);
+        atomicAdd(&fetched[1], my_tris);
+        __syncthreads();
+        if (threadIdx.x < 2)
+            atomicAdd(reinterpret_cast<unsigned long long*>(flags + 4) + threadIdx.x, fetched[threadIdx.x]);
+    }
+}
+
+// Persistent variant of k_extend (the throughput path).
+template <bool FETCH>
+__global__ __launch_bounds__(kBlock) void k_extend_p(RenderArgs a, Paths p, const uint32_t* __restrict__ q, uint32_t n,
+                                                     uint32_t* __restrict__ spill, uint32_t* __restrict__ flags,
+                                                     uint32_t sort_keys) {
+    __shared__ uint32_t lds[kLdsStack * kBlock];
+    __shared__ unsigned long long fetched[2];
+    const uint32_t gthreads = gridDim.x * kBlock;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    Stack st{lds + threadIdx.x, spill + gtid, gthreads, a.sc.stack_cap};
+    bool overflow = false;
+    unsigned long long my_nodes = 0, my_tris = 0;
+    if (FETCH) {
+        if (threadIdx.x < 2) fetched[threadIdx.x] = 0;
+        __syncthreads();
+    }
+    trace_persistent<FETCH>(
+        a.sc, n, flags + 9, st, overflow,
+        [&](uint32_t i, V3& o, V3& d) {
+            const uint32_t s = q[i];
+            const float4 o4 = p.o[s], d4 = p.d[s];
+            o = V3{o4.x, o4.y, o4.z};
+            d = V3{d4.x, d4.y, d4.z};
+        },
+        [&](uint32_t i, const Hit& h, uint32_t nn, uint32_t nt) {
+            const uint32_t s = q[i];
+            p.hit[s] = make_float4(h.t, __uint_as_float(h.idx), h.u, h.v);
+            const uint32_t key = (h.idx == kNoHit) ? a.sc.n_mats : h.mat;
+            p.mat[s] = key;
+            if (sort_keys) (void)wave_add(p.counters + 2, key);
+            if (FETCH) {
+                my_nodes += nn;
+                my_tris += nt;
+            }
+        });
     if (overflow) atomicOr(flags, 1u);
     if (FETCH) {
         atomicAdd(&fetched[0], my_nodes);
@@ -184,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Paths p, const uint32_t* __r
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = q[i];
-    const uint32_t pos = atomicAdd(&p.counters[2 + p.mat[s]], 1u);
+    const uint32_t pos = wave_add(p.counters + 2, p.mat[s]);
     out[pos] = s;
 }
 
@@ -358,7 +403,12 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     p.qs = (uint32_t*)take(cap * 4);
     p.counters = (uint32_t*)take((2 + kMaxSortKeys) * 4);
 
-    const uint32_t ext_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * 16);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_extend_p<true>),
+                                                     kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    const uint32_t ext_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)(per_cu * ctx->num_cus));
     spt_status s = ensure_spill(ctx, (size_t)ext_grid * kBlock *
                                          (ctx->stack_cap > kLdsStack ? ctx->stack_cap - kLdsStack : 1));
     if (s != SPT_OK) return s;
@@ -388,12 +438,13 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
             stats.rays += live;
             if (sort) (void)hipMemsetAsync(p.counters + 2, 0, keys * sizeof(uint32_t), st);
             const uint32_t g = std::min<uint32_t>(blocks_for(live), ext_grid);
+            (void)hipMemsetAsync(ctx->d_flags + 9, 0, 4, st);
             if (timed) (void)hipEventRecord(ctx->ev[0], st);
             if (fetch_stats)
-                hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(kBlock), 0, st, a, p, qin, live, ctx->d_spill,
+                hipLaunchKernelGGL(k_extend_p<true>, dim3(g), dim3(kBlock), 0, st, a, p, qin, live, ctx->d_spill,
                                    ctx->d_flags, sort ? 1u : 0u);
             else
-                hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(kBlock), 0, st, a, p, qin, live, ctx->d_spill,
+                hipLaunchKernelGGL(k_extend_p<false>, dim3(g), dim3(kBlock), 0, st, a, p, qin, live, ctx->d_spill,
                                    ctx->d_flags, sort ? 1u : 0u);
             if (timed) (void)hipEventRecord(ctx->ev[1], st);
             const uint32_t* qshade = qin;

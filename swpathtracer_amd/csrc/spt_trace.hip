@@ -32,6 +32,35 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, const float4* __r
     if (overflow) atomicOr(flags, 1u);
 }
 
+// Persistent closest-hit kernel: rays[i] -> hits[i] (FETCH: per-ray node and
+// triangle record counts into nodes_out / tris_out).
+template <bool FETCH>
+__global__ __launch_bounds__(kBlock) void k_trace_p(DevScene sc, const float4* __restrict__ rays, uint32_t n,
+                                                    uint2* __restrict__ hits, uint32_t* __restrict__ nodes_out,
+                                                    uint32_t* __restrict__ tris_out, uint32_t* __restrict__ spill,
+                                                    uint32_t* __restrict__ flags) {
+    __shared__ uint32_t lds[kLdsStack * kBlock];
+    const uint32_t gthreads = gridDim.x * kBlock;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    Stack st{lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap};
+    bool overflow = false;
+    trace_persistent<FETCH>(
+        sc, n, flags + 8, st, overflow,
+        [&](uint32_t i, V3& o, V3& d) {
+            const float4 ro = rays[2 * (size_t)i], rd = rays[2 * (size_t)i + 1];
+            o = V3{ro.x, ro.y, ro.z};
+            d = V3{rd.x, rd.y, rd.z};
+        },
+        [&](uint32_t i, const Hit& h, uint32_t nn, uint32_t nt) {
+            hits[i] = make_uint2(__float_as_uint(h.t), h.prim);
+            if (FETCH) {
+                nodes_out[i] = nn;
+                tris_out[i] = nt;
+            }
+        });
+    if (overflow) atomicOr(flags, 1u);
+}
+
 // One ray per pixel centre (RegularPixelSampler(1) gives (0.5, 0.5),
 // regular_pixel_sampler.cpp:3-14); value per normal_raycaster.cpp:19-31,
 // depth_raycaster.cpp:36-48, int_test_count_raycaster.cpp:75-85.
@@ -90,9 +119,36 @@ uint32_t grid_for(const spt_ctx* ctx, uint64_t n) {
 
 }  // namespace
 
+uint32_t persistent_grid(spt_ctx* ctx, const void* kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+    return (uint32_t)(per_cu * ctx->num_cus);
+}
+
 spt_status launch_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits, uint32_t mode,
                         uint32_t* d_tests, uint32_t* d_isects) {
     if (n == 0) return SPT_OK;
+    if ((mode == SPT_TRACE_CLOSEST || mode == SPT_TRACE_FETCH) && n < 0xFFFFFF00ull) {
+        const void* kern = (mode == SPT_TRACE_FETCH) ? reinterpret_cast<const void*>(&k_trace_p<true>)
+                                                    : reinterpret_cast<const void*>(&k_trace_p<false>);
+        const uint32_t grid = persistent_grid(ctx, kern);
+        spt_status s =
+            ensure_spill(ctx, (size_t)grid * kBlock * (ctx->stack_cap > kLdsStack ? ctx->stack_cap - kLdsStack : 1));
+        if (s != SPT_OK) return s;
+        s = check_hip(ctx, hipMemsetAsync(ctx->d_flags + 8, 0, 4, ctx->stream), "ray counter");
+        if (s != SPT_OK) return s;
+        DevScene sc = ctx->scene();
+        const float4* rays = reinterpret_cast<const float4*>(d_rays);
+        uint2* hits = reinterpret_cast<uint2*>(d_hits);
+        if (mode == SPT_TRACE_FETCH)
+            hipLaunchKernelGGL(k_trace_p<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, (uint32_t)n, hits,
+                               d_tests, d_isects, ctx->d_spill, ctx->d_flags);
+        else
+            hipLaunchKernelGGL(k_trace_p<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, (uint32_t)n,
+                               hits, nullptr, nullptr, ctx->d_spill, ctx->d_flags);
+        return check_hip(ctx, hipGetLastError(), "trace launch");
+    }
     const uint32_t grid = grid_for(ctx, n);
     spt_status s = ensure_spill(ctx, (size_t)grid * kBlock * (ctx->stack_cap > kLdsStack ? ctx->stack_cap - kLdsStack : 1));
     if (s != SPT_OK) return s;
