@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--no-closest-hit", action="store_true")
     ap.add_argument("--rings", type=int, default=512)
     ap.add_argument("--segments", type=int, default=1024)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); "
@@ -117,6 +119,17 @@ def main():
     from swpathtracer_amd import api, scenes
     import swpathtracer_amd._lib as L
     import ctypes as C
+
+    if args.copy_probe:
+        dev = api.Device(0)
+        nbytes = 256 << 20
+        a, b = dev.alloc(nbytes), dev.alloc(nbytes)
+        a.zero()
+        for _ in range(3):
+            dev._check(dev.lib.spt_memcpy_d2d(dev.h, b.ptr, a.ptr, nbytes))
+        dev.sync()
+        print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
+        return
 
     t0 = time.perf_counter()
     sc = scenes.mesh_in_box(rings=args.rings, segments=args.segments)

@@ -233,6 +233,7 @@ spt_status spt_malloc(spt_ctx* ctx, size_t bytes, void** d_ptr);
 spt_status spt_free(spt_ctx* ctx, void* d_ptr);
 spt_status spt_memcpy_h2d(spt_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 spt_status spt_memcpy_d2h(spt_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+spt_status spt_memcpy_d2d(spt_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 
 /* Uploads a built tree (copied). normals: n_prims x 9 floats (n0 n1 n2, smooth
  * triangle, primitives/smooth_triangle.h:11-13) or NULL for flat triangles

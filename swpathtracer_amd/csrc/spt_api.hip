@@ -210,6 +210,11 @@ spt_status spt_memcpy_d2h(spt_ctx* ctx, void* dst, const void* d_src, size_t byt
     return check_hip(ctx, hipStreamSynchronize(ctx->stream), "d2h sync");
 }
 
+spt_status spt_memcpy_d2d(spt_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    if (!ctx || (!d_dst && bytes) || (!d_src && bytes)) return SPT_ERR_INVALID;
+    return check_hip(ctx, hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream), "d2d");
+}
+
 spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_nodes, const spt_tri48* tris,
                             uint32_t n_tris, const float* root_box, const float* shading) {
     if (!ctx || !tris || n_tris == 0 || !root_box || (n_nodes && !nodes)) return SPT_ERR_INVALID;
