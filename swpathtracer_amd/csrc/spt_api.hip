@@ -153,6 +153,7 @@ void spt_destroy(spt_ctx* ctx) {
     if (ctx->d_paths) (void)hipFree(ctx->d_paths);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -385,6 +385,8 @@ struct spt_ctx {
     uint32_t* d_flags = nullptr;
     // HIP events: [0..1] bracket extend launches, [2..9] user slots (spt_event_*)
     hipEvent_t ev[10] = {};
+    // event pairs bracketing the extend launches of one render call
+    std::vector<hipEvent_t> tev;
     // path-tracing state (grown on demand)
     void* d_paths = nullptr;
     size_t paths_bytes = 0;

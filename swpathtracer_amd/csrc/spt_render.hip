@@ -17,27 +17,33 @@ namespace spt {
 
 namespace {
 
-// Path state, structure of arrays (one slot per in-flight path).
-struct Paths {
-    float4* o;      // ray origin
-    float4* d;      // ray direction
-    float4* thr;    // throughput
-    float4* rad;    // radiance so far
-    float4* hit;    // t, leaf index (bits), u, v
-    uint64_t* key;  // RNG key or replay row
-    uint64_t* local;  // path index within this render call
-    uint32_t* pix;  // pixel index (y * width + x)
-    uint32_t* k;    // draws consumed
-    uint32_t* depth;
-    uint32_t* mat;  // sort key of the current hit (n_mats = miss)
-    uint32_t* q0;
-    uint32_t* q1;
-    uint32_t* qs;
-    uint32_t* counters;  // [0] queue length out, [1] stream overflow count, [2..] material histogram
-};
+constexpr uint32_t kMaxSortKeys = 1024;  // materials + 1 (miss) sorted; more keys: unsorted
+constexpr uint32_t kSortBlocks = 256;     // blocks of the histogram / scatter passes
+constexpr uint32_t kScanThreads = 1024;
 
-constexpr size_t kSlotBytes = 16 * 5 + 8 * 2 + 4 * 8;
-constexpr uint32_t kMaxSortKeys = 1024;
+// Path state of one buffer (structure of arrays, one slot per live path).
+// Two buffers alternate: shade reads buffer A and appends survivors to B.
+struct PathBuf {
+    float4* o;       // ray origin
+    float4* d;       // ray direction
+    float4* thr;     // throughput
+    float4* rad;     // radiance so far
+    uint4* meta;     // pixel index, draws consumed, depth, unused
+    uint64_t* key;   // RNG key (counter mode) or replay-stream row
+    uint64_t* local; // path index within this render call (per-path output)
+};
+constexpr size_t kBufBytes = 16 * 5 + 8 * 2;
+
+// Per-bounce scratch shared by both buffers.
+struct Work {
+    float4* hit;       // t, leaf-order triangle index (bits), u, v   (indexed like buffer A)
+    uint32_t* skey;    // sort key: material of the hit, n_mats for a miss
+    uint32_t* perm;    // shading order (stable counting sort by skey)
+    uint32_t* bhist;   // [key][block] histogram, then exclusive offsets
+    uint32_t* cnt;     // [0], [1]: live counts of the two buffers; [2]: replay overflows;
+                       // [8]: ray-fetch counter of the extend kernel; u64 at [10]: rays, u64 at [12]: bounces
+};
+constexpr size_t kWorkBytes = 16 + 4 + 4;
 
 struct RenderArgs {
     DevScene sc;
@@ -54,129 +60,98 @@ struct RenderArgs {
     float* film32;
     double* film64;
     float* path_rgb;
+    uint32_t keys;  // sort keys (n_mats + 1), 0 = no sort
 };
 
-__device__ inline void finish_path(const RenderArgs& a, const Paths& p, uint32_t s, float4 L) {
+__device__ inline void finish_path(const RenderArgs& a, uint32_t pix, uint64_t local, float4 L) {
     if (a.film32) {
-        float* f = a.film32 + 3 * (size_t)p.pix[s];
+        float* f = a.film32 + 3 * (size_t)pix;
         unsafeAtomicAdd(f + 0, L.x);
         unsafeAtomicAdd(f + 1, L.y);
         unsafeAtomicAdd(f + 2, L.z);
     }
     if (a.film64) {
-        double* f = a.film64 + 3 * (size_t)p.pix[s];
+        double* f = a.film64 + 3 * (size_t)pix;
         unsafeAtomicAdd(f + 0, (double)L.x);
         unsafeAtomicAdd(f + 1, (double)L.y);
         unsafeAtomicAdd(f + 2, (double)L.z);
     }
     if (a.path_rgb) {
-        float* o = a.path_rgb + 3 * (size_t)p.local[s];
+        float* o = a.path_rgb + 3 * (size_t)local;
         o[0] = L.x;
         o[1] = L.y;
         o[2] = L.z;
     }
 }
 
-__device__ inline Draws make_draws(const RenderArgs& a, const Paths& p, uint32_t s) {
+__device__ inline Draws make_draws(const RenderArgs& a, uint64_t key, uint32_t k) {
     Draws r;
-    r.k = p.k[s];
+    r.k = k;
     r.overflow = false;
     r.stride = a.stride;
     if (a.stream) {
         r.key = 0;
-        r.stream = a.stream + p.key[s] * (uint64_t)a.stride;
+        r.stream = a.stream + key * (uint64_t)a.stride;
     } else {
-        r.key = p.key[s];
+        r.key = key;
         r.stream = nullptr;
     }
     return r;
 }
 
-// Camera samples for paths [base, base + n): pathtracer.cpp:104-113.
-__global__ __launch_bounds__(kBlock) void k_generate(RenderArgs a, Paths p, uint64_t base, uint32_t n) {
+// Camera samples for paths [base, base + n) into buffer A:
+// integrators/pathtracer.cpp:104-113.
+__global__ __launch_bounds__(kBlock) void k_generate(RenderArgs a, PathBuf A, Work w, uint64_t base, uint32_t n) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s == 0) {
+        w.cnt[0] = (a.max_depth == 0) ? 0u : n;
+        w.cnt[1] = 0;
+        w.cnt[8] = 0;
+    }
     if (s >= n) return;
     const uint64_t g = base + s;
     const uint64_t pl = g / a.spp_count;
     const uint32_t smp = a.spp_begin + (uint32_t)(g % a.spp_count);
     const uint32_t x = a.vl + (uint32_t)(pl % a.vw), y = a.vt + (uint32_t)(pl / a.vw);
     const uint64_t path = ((uint64_t)y * a.cam.width + x) * a.spp + smp;
-    p.pix[s] = y * a.cam.width + x;
-    p.local[s] = g;
-    p.key[s] = a.stream ? g : rng_key(a.seed, path);
-    p.k[s] = 0;
-    Draws r = make_draws(a, p, s);
+    const uint32_t pix = y * a.cam.width + x;
+    const uint64_t key = a.stream ? g : rng_key(a.seed, path);
+    Draws r = make_draws(a, key, 0);
     float sx, sy;
     pixel_offset(a.sampler, a.n_sub, a.w_sub, smp, r, sx, sy);
     const float xs = x + sx, ys = y + sy;
     V3 o, d;
     camera_ray(a.cam, xs, ys, o, d);
-    p.k[s] = r.k;
-    if (r.overflow) atomicAdd(&p.counters[1], 1u);
-    p.o[s] = make_float4(o.x, o.y, o.z, 0.0f);
-    p.d[s] = make_float4(d.x, d.y, d.z, 0.0f);
-    p.thr[s] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    p.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    p.depth[s] = 0;
+    if (r.overflow) atomicAdd(&w.cnt[2], 1u);
     if (a.max_depth == 0) {  // depth >= path_length before the first trace
-        finish_path(a, p, s, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        finish_path(a, pix, g, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         return;
     }
-    const uint32_t slot = atomicAdd(&p.counters[0], 1u);
-    p.q1[slot] = s;
+    A.o[s] = make_float4(o.x, o.y, o.z, 0.0f);
+    A.d[s] = make_float4(d.x, d.y, d.z, 0.0f);
+    A.thr[s] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    A.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    A.meta[s] = make_uint4(pix, r.k, 0u, 0u);
+    A.key[s] = key;
+    A.local[s] = g;
 }
 
-// Closest hit for every queued path. FETCH adds the node / triangle record
-// fetch counts of all rays to the two 64-bit counters at flags + 4.
+// Closest hit of every live path of buffer A (count *cntA). Persistent
+// traversal; hit record and sort key written at the path's slot. FETCH adds
+// node / triangle record fetch counts to the two 64-bit counters at flags + 4.
 template <bool FETCH>
-__global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, Paths p, const uint32_t* __restrict__ q, uint32_t n,
-                                                   uint32_t* __restrict__ spill, uint32_t* __restrict__ flags,
-                                                   uint32_t sort_keys) {
+__global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, PathBuf A, Work w, const uint32_t* __restrict__ cntA,
+                                                   uint32_t* __restrict__ spill, uint32_t* __restrict__ flags) {
     __shared__ uint32_t lds[kLdsStack * kBlock];
     __shared__ unsigned long long fetched[2];
+    const uint32_t n = *cntA;
+    if (n == 0) return;
     const uint32_t gthreads = gridDim.x * kBlock;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    Stack st{lds + threadIdx.x, spill + gtid, gthreads, a.sc.stack_cap};
-    bool overflow = false;
-    unsigned long long my_nodes = 0, my_tris = 0;
-    if (FETCH) {
-        if (threadIdx.x < 2) fetched[threadIdx.x] = 0;
-        __syncthreads();
+    if (gtid == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(w.cnt + 10), (unsigned long long)n);
+        atomicAdd(reinterpret_cast<unsigned long long*>(w.cnt + 12), 1ull);
     }
-    for (uint32_t i = gtid; i < n; i += gthreads) {
-        const uint32_t s = q[i];
-        const float4 o = p.o[s], d = p.d[s];
-        Hit h;
-        uint32_t tests, isects;
-        trace_ray<true, false, FETCH>(a.sc, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, st, h, tests, isects, overflow);
-        if (FETCH) {
-            my_nodes += tests;
-            my_tris += isects;
-        }
-        p.hit[s] = make_float4(h.t, __uint_as_float(h.idx), h.u, h.v);
-        const uint32_t key = (h.idx == kNoHit) ? a.sc.n_mats : h.mat;
-        p.mat[s] = key;
-        if (sort_keys) (void)wave_add(p.counters + 2, key);
-    }
-    if (overflow) atomicOr(flags, 1u);
-    if (FETCH) {
-        atomicAdd(&fetched[0], my_nodes);
-        atomicAdd(&fetched[1], my_tris);
-        __syncthreads();
-        if (threadIdx.x < 2)
-            atomicAdd(reinterpret_cast<unsigned long long*>(flags + 4) + threadIdx.x, fetched[threadIdx.x]);
-    }
-}
-
-// Persistent variant of k_extend (the throughput path).
-template <bool FETCH>
-__global__ __launch_bounds__(kBlock) void k_extend_p(RenderArgs a, Paths p, const uint32_t* __restrict__ q, uint32_t n,
-                                                     uint32_t* __restrict__ spill, uint32_t* __restrict__ flags,
-                                                     uint32_t sort_keys) {
-    __shared__ uint32_t lds[kLdsStack * kBlock];
-    __shared__ unsigned long long fetched[2];
-    const uint32_t gthreads = gridDim.x * kBlock;
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     Stack st{lds + threadIdx.x, spill + gtid, gthreads, a.sc.stack_cap};
     bool overflow = false;
     unsigned long long my_nodes = 0, my_tris = 0;
@@ -185,19 +160,15 @@ __global__ __launch_bounds__(kBlock) void k_extend_p(RenderArgs a, Paths p, cons
         __syncthreads();
     }
     trace_persistent<FETCH>(
-        a.sc, n, flags + 9, st, overflow,
+        a.sc, n, w.cnt + 8, st, overflow,
         [&](uint32_t i, V3& o, V3& d) {
-            const uint32_t s = q[i];
-            const float4 o4 = p.o[s], d4 = p.d[s];
+            const float4 o4 = A.o[i], d4 = A.d[i];
             o = V3{o4.x, o4.y, o4.z};
             d = V3{d4.x, d4.y, d4.z};
         },
         [&](uint32_t i, const Hit& h, uint32_t nn, uint32_t nt) {
-            const uint32_t s = q[i];
-            p.hit[s] = make_float4(h.t, __uint_as_float(h.idx), h.u, h.v);
-            const uint32_t key = (h.idx == kNoHit) ? a.sc.n_mats : h.mat;
-            p.mat[s] = key;
-            if (sort_keys) (void)wave_add(p.counters + 2, key);
+            w.hit[i] = make_float4(h.t, __uint_as_float(h.idx), h.u, h.v);
+            w.skey[i] = (h.idx == kNoHit) ? a.sc.n_mats : h.mat;
             if (FETCH) {
                 my_nodes += nn;
                 my_tris += nt;
@@ -213,95 +184,180 @@ __global__ __launch_bounds__(kBlock) void k_extend_p(RenderArgs a, Paths p, cons
     }
 }
 
-// Exclusive scan of the material histogram (one block).
-__global__ void k_scan(Paths p, uint32_t keys) {
-    if (threadIdx.x != 0) return;
-    uint32_t acc = 0;
-    for (uint32_t k = 0; k < keys; ++k) {
-        const uint32_t c = p.counters[2 + k];
-        p.counters[2 + k] = acc;
+// Stable counting sort of the live slots by material, in three passes over
+// kSortBlocks contiguous chunks: per-chunk histogram (LDS), one exclusive scan
+// over [key][chunk], then a scatter that ranks equal keys in slot order. No
+// global atomics: many lanes with one material would serialise on them.
+__device__ inline void chunk_range(uint32_t n, uint32_t b, uint32_t& c0, uint32_t& c1) {
+    const uint32_t per = (n + kSortBlocks - 1) / kSortBlocks;
+    c0 = min(n, b * per);
+    c1 = min(n, c0 + per);
+}
+
+__global__ __launch_bounds__(kBlock) void k_hist(RenderArgs a, Work w, const uint32_t* __restrict__ cntA) {
+    __shared__ uint32_t hist[kMaxSortKeys];
+    const uint32_t n = *cntA;
+    for (uint32_t k = threadIdx.x; k < a.keys; k += kBlock) hist[k] = 0;
+    __syncthreads();
+    uint32_t c0, c1;
+    chunk_range(n, blockIdx.x, c0, c1);
+    for (uint32_t e = c0 + threadIdx.x; e < c1; e += kBlock) atomicAdd(&hist[w.skey[e]], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < a.keys; k += kBlock) w.bhist[k * kSortBlocks + blockIdx.x] = hist[k];
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan(RenderArgs a, Work w) {
+    __shared__ uint32_t part[kScanThreads];
+    const uint32_t total = a.keys * kSortBlocks;
+    const uint32_t per = (total + kScanThreads - 1) / kScanThreads;
+    const uint32_t b0 = min(total, threadIdx.x * per), b1 = min(total, b0 + per);
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b1; ++i) sum += w.bhist[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t v = (threadIdx.x >= off) ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t acc = part[threadIdx.x] - sum;
+    for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t c = w.bhist[i];
+        w.bhist[i] = acc;
         acc += c;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(Paths p, const uint32_t* __restrict__ q, uint32_t n,
-                                                    uint32_t* __restrict__ out) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = q[i];
-    const uint32_t pos = wave_add(p.counters + 2, p.mat[s]);
-    out[pos] = s;
+__global__ __launch_bounds__(kBlock) void k_scatter(RenderArgs a, Work w, const uint32_t* __restrict__ cntA) {
+    __shared__ uint32_t base[kMaxSortKeys];
+    __shared__ uint32_t wcnt[kBlock / 64][kMaxSortKeys];
+    const uint32_t n = *cntA;
+    const uint32_t wave = threadIdx.x / 64;
+    for (uint32_t k = threadIdx.x; k < a.keys; k += kBlock) {
+        base[k] = w.bhist[k * kSortBlocks + blockIdx.x];
+        for (uint32_t v = 0; v < kBlock / 64; ++v) wcnt[v][k] = 0;
+    }
+    __syncthreads();
+    uint32_t c0, c1;
+    chunk_range(n, blockIdx.x, c0, c1);
+    const uint64_t lt = __lanemask_lt();
+    for (uint32_t r0 = c0; r0 < c1; r0 += kBlock) {
+        const uint32_t e = r0 + threadIdx.x;
+        const bool valid = e < c1;
+        const uint32_t key = valid ? w.skey[e] : 0u;
+        uint32_t rank = 0;
+        uint64_t todo = __ballot(valid);
+        while (todo) {  // rank among equal keys of this wave, count per key
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const uint32_t lkey = __shfl(key, leader);
+            const uint64_t group = __ballot(valid && key == lkey) & todo;
+            if (__lane_id() == (unsigned)leader) wcnt[wave][lkey] = (uint32_t)__popcll(group);
+            if (valid && key == lkey) rank = (uint32_t)__popcll(group & lt);
+            todo &= ~group;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[key] + rank;
+            for (uint32_t v = 0; v < wave; ++v) pos += wcnt[v][key];
+            w.perm[pos] = e;
+        }
+        __syncthreads();
+        todo = __ballot(valid);
+        while (todo) {  // advance the running bases, clear this wave's counts
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const uint32_t lkey = __shfl(key, leader);
+            const uint64_t group = __ballot(valid && key == lkey) & todo;
+            if (__lane_id() == (unsigned)leader) {
+                atomicAdd(&base[lkey], wcnt[wave][lkey]);
+                wcnt[wave][lkey] = 0;
+            }
+            todo &= ~group;
+        }
+        __syncthreads();
+    }
 }
 
-// Shading of one bounce: pathtracer.cpp:21-54 for every queued path.
-__global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, Paths p, const uint32_t* __restrict__ q, uint32_t n,
-                                                  uint32_t* __restrict__ qout) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = q[i];
-    const float4 hv = p.hit[s];
-    float4 rad = p.rad[s];
-    const float4 thr = p.thr[s];
-    const uint32_t depth = p.depth[s];
-    const uint32_t idx = __float_as_uint(hv.y);
-    if (idx == kNoHit) {  // miss: background at depth 0, else 0 (pathtracer.cpp:21-26)
-        if (depth == 0) {
-            rad.x += thr.x * a.bg[0];
-            rad.y += thr.y * a.bg[1];
-            rad.z += thr.z * a.bg[2];
+// Shading of one bounce (integrators/pathtracer.cpp:21-54) for every live
+// path of A, in material order when sorted; survivors are appended to B.
+__global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathBuf B, Work w,
+                                                  const uint32_t* __restrict__ cntA, uint32_t* __restrict__ cntB) {
+    const uint32_t n = *cntA;
+    const uint32_t gthreads = gridDim.x * kBlock;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gthreads) {
+        const uint32_t i = a.keys ? w.perm[j] : j;
+        const float4 hv = w.hit[i];
+        float4 rad = A.rad[i];
+        const float4 thr = A.thr[i];
+        const uint4 meta = A.meta[i];
+        const uint32_t depth = meta.z;
+        const uint32_t idx = __float_as_uint(hv.y);
+        if (idx == kNoHit) {  // miss: background at depth 0, else 0 (pathtracer.cpp:21-26)
+            if (depth == 0) {
+                rad.x += thr.x * a.bg[0];
+                rad.y += thr.y * a.bg[1];
+                rad.z += thr.z * a.bg[2];
+            }
+            finish_path(a, meta.x, A.local[i], rad);
+            continue;
         }
-        finish_path(a, p, s, rad);
-        return;
-    }
-    const float4 o4 = p.o[s], d4 = p.d[s];
-    const V3 o{o4.x, o4.y, o4.z}, d{d4.x, d4.y, d4.z};
-    Hit h;
-    h.t = hv.x;
-    h.idx = idx;
-    h.u = hv.z;
-    h.v = hv.w;
-    const V3 pos = o + d * h.t;  // triangle.cpp:69
-    const V3 nrm = shading_normal(a.sc, h);
-    const uint32_t mid = p.mat[s];
-    const spt_material& m = a.sc.mats[mid];
-    const Frame fr = frame_from_up(nrm);              // pathtracer.cpp:28
-    const V3 wo = normalize(to_local(fr, -d));        // pathtracer.cpp:30
-    Draws r = make_draws(a, p, s);
-    V3 bsdf, refl, wi;
-    float pdf;
-    layered_eval(m, wo, bsdf, refl, pdf, wi, r);      // pathtracer.cpp:36
-    const V3 wwi = to_world(fr, wi);                  // pathtracer.cpp:37
-    const V3 no = pos + wwi * 1e-3f;                  // pathtracer.cpp:38
-    rad.x += thr.x * m.emission[0];
-    rad.y += thr.y * m.emission[1];
-    rad.z += thr.z * m.emission[2];
-    const float kk = gmax(gmax(refl.x, refl.y), refl.z);  // pathtracer.cpp:41
-    if ((a.rr_depth >= 0) && ((int32_t)depth >= a.rr_depth)) {  // pathtracer.cpp:43-49
-        if ((r.next() < kk) && (depth < 1000u)) {
-            bsdf = bsdf / kk;
-        } else {
-            if (r.overflow) atomicAdd(&p.counters[1], 1u);
-            finish_path(a, p, s, rad);
-            return;
+        const float4 o4 = A.o[i], d4 = A.d[i];
+        const V3 o{o4.x, o4.y, o4.z}, d{d4.x, d4.y, d4.z};
+        Hit h;
+        h.t = hv.x;
+        h.idx = idx;
+        h.u = hv.z;
+        h.v = hv.w;
+        const V3 pos = o + d * h.t;  // triangle.cpp:69
+        const V3 nrm = shading_normal(a.sc, h);
+        const spt_material& m = a.sc.mats[w.skey[i]];
+        const Frame fr = frame_from_up(nrm);          // pathtracer.cpp:28
+        const V3 wo = normalize(to_local(fr, -d));    // pathtracer.cpp:30
+        const uint64_t key = A.key[i];
+        Draws r = make_draws(a, key, meta.y);
+        V3 bsdf, refl, wi;
+        float pdf;
+        layered_eval(m, wo, bsdf, refl, pdf, wi, r);  // pathtracer.cpp:36
+        const V3 wwi = to_world(fr, wi);              // pathtracer.cpp:37
+        const V3 no = pos + wwi * 1e-3f;              // pathtracer.cpp:38
+        rad.x += thr.x * m.emission[0];
+        rad.y += thr.y * m.emission[1];
+        rad.z += thr.z * m.emission[2];
+        const float kk = gmax(gmax(refl.x, refl.y), refl.z);  // pathtracer.cpp:41
+        bool alive = true;
+        if ((a.rr_depth >= 0) && ((int32_t)depth >= a.rr_depth)) {  // pathtracer.cpp:43-49
+            if ((r.next() < kk) && (depth < 1000u))
+                bsdf = bsdf / kk;
+            else
+                alive = false;
         }
+        if (r.overflow) atomicAdd(&w.cnt[2], 1u);
+        const uint32_t nd = depth + 1;
+        if (alive && (a.max_depth >= 0) && ((int32_t)nd >= a.max_depth)) alive = false;  // pathtracer.cpp:18
+        if (!alive) {
+            finish_path(a, meta.x, A.local[i], rad);
+            continue;
+        }
+        // pathtracer.cpp:52-54, forwards: throughput *= f * cos / pdf
+        const V3 wgt = (bsdf * wi.y) / pdf;
+        const uint32_t slot = atomicAdd(cntB, 1u);
+        B.o[slot] = make_float4(no.x, no.y, no.z, 0.0f);
+        B.d[slot] = make_float4(wwi.x, wwi.y, wwi.z, 0.0f);
+        B.thr[slot] = make_float4(thr.x * wgt.x, thr.y * wgt.y, thr.z * wgt.z, 0.0f);
+        B.rad[slot] = rad;
+        B.meta[slot] = make_uint4(meta.x, r.k, nd, 0u);
+        B.key[slot] = key;
+        B.local[slot] = A.local[i];
     }
-    if (r.overflow) atomicAdd(&p.counters[1], 1u);
-    // pathtracer.cpp:52-54, forwards: throughput *= f * cos / pdf
-    const V3 wgt = (bsdf * wi.y) / pdf;
-    const float4 nthr = make_float4(thr.x * wgt.x, thr.y * wgt.y, thr.z * wgt.z, 0.0f);
-    const uint32_t nd = depth + 1;
-    if ((a.max_depth >= 0) && ((int32_t)nd >= a.max_depth)) {  // pathtracer.cpp:18
-        finish_path(a, p, s, rad);
-        return;
+}
+
+// End of a bounce: the consumed buffer's count and the ray-fetch counter go
+// back to zero (a kernel, not a memset, so the bounce can be graph-captured).
+__global__ void k_flip(uint32_t* cntA, uint32_t* fetch) {
+    if (threadIdx.x == 0) {
+        *cntA = 0;
+        *fetch = 0;
     }
-    p.rad[s] = rad;
-    p.thr[s] = nthr;
-    p.depth[s] = nd;
-    p.k[s] = r.k;
-    p.o[s] = make_float4(no.x, no.y, no.z, 0.0f);
-    p.d[s] = make_float4(wwi.x, wwi.y, wwi.z, 0.0f);
-    const uint32_t slot = atomicAdd(&p.counters[0], 1u);
-    qout[slot] = s;
 }
 
 uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
@@ -366,11 +422,12 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     }
     const uint64_t total = (uint64_t)a.vw * a.vh * a.spp_count;
     const uint32_t keys = ctx->n_mats + 1;
-    const bool sort = !(prm->flags & SPT_RENDER_NO_SORT) && keys <= kMaxSortKeys;
+    a.keys = (!(prm->flags & SPT_RENDER_NO_SORT) && keys <= kMaxSortKeys) ? keys : 0u;
 
     // path slots: enough for the whole call, at most 4M in flight
     const uint64_t cap = std::min<uint64_t>(total, 1u << 22);
-    const size_t bytes = (size_t)cap * kSlotBytes + (2 + kMaxSortKeys) * sizeof(uint32_t) + 1024;
+    const size_t bytes = (size_t)cap * (2 * kBufBytes + kWorkBytes + 4 * 8) +
+                         (size_t)kMaxSortKeys * kSortBlocks * 4 + 4096;
     if (bytes > ctx->paths_bytes) {
         (void)hipStreamSynchronize(ctx->stream);
         if (ctx->d_paths) (void)hipFree(ctx->d_paths);
@@ -380,97 +437,119 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
         if (s != SPT_OK) return s;
         ctx->paths_bytes = bytes;
     }
-    Paths p;
     char* base = reinterpret_cast<char*>(ctx->d_paths);
     auto take = [&](size_t sz) {
         char* r = base;
         base += (sz + 255) & ~size_t(255);
         return r;
     };
-    p.o = (float4*)take(cap * 16);
-    p.d = (float4*)take(cap * 16);
-    p.thr = (float4*)take(cap * 16);
-    p.rad = (float4*)take(cap * 16);
-    p.hit = (float4*)take(cap * 16);
-    p.key = (uint64_t*)take(cap * 8);
-    p.local = (uint64_t*)take(cap * 8);
-    p.pix = (uint32_t*)take(cap * 4);
-    p.k = (uint32_t*)take(cap * 4);
-    p.depth = (uint32_t*)take(cap * 4);
-    p.mat = (uint32_t*)take(cap * 4);
-    p.q0 = (uint32_t*)take(cap * 4);
-    p.q1 = (uint32_t*)take(cap * 4);
-    p.qs = (uint32_t*)take(cap * 4);
-    p.counters = (uint32_t*)take((2 + kMaxSortKeys) * 4);
+    PathBuf buf[2];
+    for (PathBuf& b : buf) {
+        b.o = (float4*)take(cap * 16);
+        b.d = (float4*)take(cap * 16);
+        b.thr = (float4*)take(cap * 16);
+        b.rad = (float4*)take(cap * 16);
+        b.meta = (uint4*)take(cap * 16);
+        b.key = (uint64_t*)take(cap * 8);
+        b.local = (uint64_t*)take(cap * 8);
+    }
+    Work w;
+    w.hit = (float4*)take(cap * 16);
+    w.skey = (uint32_t*)take(cap * 4);
+    w.perm = (uint32_t*)take(cap * 4);
+    w.bhist = (uint32_t*)take((size_t)kMaxSortKeys * kSortBlocks * 4);
+    w.cnt = (uint32_t*)take(256);
+    if ((size_t)(base - reinterpret_cast<char*>(ctx->d_paths)) > ctx->paths_bytes)
+        return set_error(ctx, SPT_ERR_NOMEM, "path state layout exceeds its allocation");
 
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_extend_p<true>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_extend<true>),
                                                      kBlock, 0) != hipSuccess ||
         per_cu <= 0)
         per_cu = 4;
     const uint32_t ext_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)(per_cu * ctx->num_cus));
+    const uint32_t shade_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * 8);
     spt_status s = ensure_spill(ctx, (size_t)ext_grid * kBlock *
                                          (ctx->stack_cap > kLdsStack ? ctx->stack_cap - kLdsStack : 1));
     if (s != SPT_OK) return s;
     hipStream_t st = ctx->stream;
     const bool timed = (prm->flags & SPT_RENDER_TIME_KERNELS) != 0;
     const bool fetch_stats = (prm->flags & SPT_RENDER_FETCH_STATS) != 0;
+    s = check_hip(ctx, hipMemsetAsync(w.cnt, 0, 256, st), "counters");
+    if (s != SPT_OK) return s;
     if (fetch_stats) {
         s = check_hip(ctx, hipMemsetAsync(ctx->d_flags + 4, 0, 16, st), "fetch counters");
         if (s != SPT_OK) return s;
     }
+    // bounces are issued without waiting; the live count is read every kCheck bounces
+    constexpr uint32_t kCheck = 4;
     spt_render_stats stats{};
     stats.paths = total;
-    uint32_t overflow_total = 0;
     for (uint64_t done = 0; done < total; done += cap) {
         const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - done);
-        s = check_hip(ctx, hipMemsetAsync(p.counters, 0, 2 * sizeof(uint32_t), st), "counters");
+        hipLaunchKernelGGL(k_generate, dim3(blocks_for(n)), dim3(kBlock), 0, st, a, buf[0], w, done, n);
+        s = check_hip(ctx, hipGetLastError(), "generate launch");
         if (s != SPT_OK) return s;
-        hipLaunchKernelGGL(k_generate, dim3(blocks_for(n)), dim3(kBlock), 0, st, a, p, done, n);
-        uint32_t live = 0;
-        s = check_hip(ctx, hipMemcpyAsync(&live, p.counters, 4, hipMemcpyDeviceToHost, st), "count");
-        if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "generate");
-        if (s != SPT_OK) return s;
-        uint32_t* qin = p.q1;
-        uint32_t* qout = p.q0;
-        while (live > 0) {
-            ++stats.bounces;
-            stats.rays += live;
-            if (sort) (void)hipMemsetAsync(p.counters + 2, 0, keys * sizeof(uint32_t), st);
-            const uint32_t g = std::min<uint32_t>(blocks_for(live), ext_grid);
-            (void)hipMemsetAsync(ctx->d_flags + 9, 0, 4, st);
-            if (timed) (void)hipEventRecord(ctx->ev[0], st);
-            if (fetch_stats)
-                hipLaunchKernelGGL(k_extend_p<true>, dim3(g), dim3(kBlock), 0, st, a, p, qin, live, ctx->d_spill,
-                                   ctx->d_flags, sort ? 1u : 0u);
-            else
-                hipLaunchKernelGGL(k_extend_p<false>, dim3(g), dim3(kBlock), 0, st, a, p, qin, live, ctx->d_spill,
-                                   ctx->d_flags, sort ? 1u : 0u);
-            if (timed) (void)hipEventRecord(ctx->ev[1], st);
-            const uint32_t* qshade = qin;
-            if (sort) {
-                hipLaunchKernelGGL(k_scan, dim3(1), dim3(64), 0, st, p, keys);
-                hipLaunchKernelGGL(k_scatter, dim3(blocks_for(live)), dim3(kBlock), 0, st, p, qin, live, p.qs);
-                qshade = p.qs;
-            }
-            (void)hipMemsetAsync(p.counters, 0, sizeof(uint32_t), st);
-            hipLaunchKernelGGL(k_shade, dim3(blocks_for(live)), dim3(kBlock), 0, st, a, p, qshade, live, qout);
-            s = check_hip(ctx, hipMemcpyAsync(&live, p.counters, 4, hipMemcpyDeviceToHost, st), "count");
-            if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "bounce");
-            if (s != SPT_OK) return s;
+        if (a.max_depth == 0) continue;
+        uint32_t b = 0;
+        while (true) {
+            PathBuf& A = buf[b & 1];
+            PathBuf& B = buf[(b + 1) & 1];
+            uint32_t* cntA = w.cnt + (b & 1);
+            uint32_t* cntB = w.cnt + ((b + 1) & 1);
             if (timed) {
-                float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) stats.extend_ms += ms;
+                while (ctx->tev.size() < 2 * (stats.extend_launches + 1)) {
+                    hipEvent_t e;
+                    s = check_hip(ctx, hipEventCreate(&e), "event");
+                    if (s != SPT_OK) return s;
+                    ctx->tev.push_back(e);
+                }
+                (void)hipEventRecord(ctx->tev[2 * stats.extend_launches], st);
+            }
+            if (fetch_stats)
+                hipLaunchKernelGGL(k_extend<true>, dim3(ext_grid), dim3(kBlock), 0, st, a, A, w, cntA, ctx->d_spill,
+                                   ctx->d_flags);
+            else
+                hipLaunchKernelGGL(k_extend<false>, dim3(ext_grid), dim3(kBlock), 0, st, a, A, w, cntA, ctx->d_spill,
+                                   ctx->d_flags);
+            if (timed) {
+                (void)hipEventRecord(ctx->tev[2 * stats.extend_launches + 1], st);
                 ++stats.extend_launches;
             }
-            std::swap(qin, qout);
+            if (a.keys) {
+                hipLaunchKernelGGL(k_hist, dim3(kSortBlocks), dim3(kBlock), 0, st, a, w, cntA);
+                hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, a, w);
+                hipLaunchKernelGGL(k_scatter, dim3(kSortBlocks), dim3(kBlock), 0, st, a, w, cntA);
+            }
+            hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kBlock), 0, st, a, A, B, w, cntA, cntB);
+            hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, st, cntA, w.cnt + 8);
+            s = check_hip(ctx, hipGetLastError(), "bounce launch");
+            if (s != SPT_OK) return s;
+            ++b;
+            const bool capped = (a.max_depth >= 0) && (b >= (uint32_t)a.max_depth);
+            if (capped) break;
+            if (b % kCheck == 0) {
+                uint32_t live = 0;
+                s = check_hip(ctx, hipMemcpyAsync(&live, w.cnt + (b & 1), 4, hipMemcpyDeviceToHost, st), "count");
+                if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "bounce");
+                if (s != SPT_OK) return s;
+                if (live == 0) break;
+            }
         }
-        uint32_t ov = 0;
-        s = check_hip(ctx, hipMemcpy(&ov, p.counters + 1, 4, hipMemcpyDeviceToHost), "overflow count");
-        if (s != SPT_OK) return s;
-        overflow_total += ov;
     }
-    stats.stream_overflows = overflow_total;
+    uint32_t cnt[16];
+    s = check_hip(ctx, hipMemcpyAsync(cnt, w.cnt, sizeof(cnt), hipMemcpyDeviceToHost, st), "counters");
+    if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "render");
+    if (s != SPT_OK) return s;
+    stats.stream_overflows = cnt[2];
+    stats.rays = (uint64_t)cnt[10] | ((uint64_t)cnt[11] << 32);
+    stats.bounces = (uint64_t)cnt[12] | ((uint64_t)cnt[13] << 32);
+    if (timed) {
+        for (uint64_t i = 0; i < stats.extend_launches; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ctx->tev[2 * i], ctx->tev[2 * i + 1]) == hipSuccess) stats.extend_ms += ms;
+        }
+    }
     stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->stats = stats;
     uint32_t flags = 0;
@@ -487,7 +566,7 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
         (void)hipMemset(ctx->d_flags, 0, 4);
         return set_error(ctx, SPT_ERR_OVERFLOW, "traversal stack overflow");
     }
-    if (overflow_total) return set_error(ctx, SPT_ERR_OVERFLOW, "replay stream exhausted");
+    if (stats.stream_overflows) return set_error(ctx, SPT_ERR_OVERFLOW, "replay stream exhausted");
     return SPT_OK;
 }
 
