@@ -1167,6 +1167,8 @@ int orc_material_sample(const spt_material* m, const float* wo, uint64_t seed, u
     return SPT_OK;
 }
 
+float orc_fresnel_conductor(float cos_i, float eta, float k) { return fresnelConductor1(cos_i, eta, k); }
+
 int orc_layer_eval(const spt_layer* l, const float* wo, const float* wi, float* out) {
     if (!l || !wo || !wi || !out) return SPT_ERR_INVALID;
     Layer layer{*l};

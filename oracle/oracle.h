@@ -64,6 +64,9 @@ int orc_material_sample(const spt_material* m, const float* wo, uint64_t seed, u
  * (Lambertian, Cook-Torrance, GGX). */
 int orc_layer_eval(const spt_layer* l, const float* wo, const float* wi, float* out);
 
+/* Exact conductor Fresnel reflectance for complex IOR eta + i k (GGX layer). */
+float orc_fresnel_conductor(float cos_i, float eta, float k);
+
 /* Path tracing (integrators/pathtracer.cpp). rgb32: w*h*3 pixel values as the
  * reference leaves them (sum / spp, FP32); rgb64: same with FP64 sums; path_rgb:
  * per path radiance. params->d_stream, when used, is a HOST pointer here.

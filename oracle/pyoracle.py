@@ -46,6 +46,7 @@ def load():
         "orc_rng_u": (f, [u64, u64, u32]),
         "orc_material_sample": (i, [C.POINTER(L.Material), vp, u64, u64, vp]),
         "orc_layer_eval": (i, [C.POINTER(L.Layer), vp, vp, vp]),
+        "orc_fresnel_conductor": (f, [f, f, f]),
         "orc_render": (i, [vp, C.POINTER(L.Camera), C.POINTER(L.RenderParams), vp, vp, vp, vp, i]),
         "orc_raycast": (i, [vp, C.POINTER(L.Camera), u32, vp, i]),
     }
@@ -189,6 +190,10 @@ def material_sample(mat, wo, seed, path):
     w = np.ascontiguousarray(wo, np.float32)
     assert load().orc_material_sample(C.byref(mat), _p(w), seed, path, _p(out)) == 0
     return out
+
+
+def fresnel_conductor(cos_i, eta, k):
+    return load().orc_fresnel_conductor(cos_i, eta, k)
 
 
 def layer_eval(layer, wo, wi):
