@@ -42,11 +42,18 @@ struct Hit {
 // Traversal stack: the first kLdsStack entries of a lane live in LDS
 // (layout [entry][lane], conflict-free), deeper entries spill to a global
 // buffer laid out [entry - kLdsStack][thread] so a wave's accesses coalesce.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(1))) uint32_t glb_u32;
+
 struct Stack {
-    uint32_t* lds;      // &lds[threadIdx.x], stride kBlock
-    uint32_t* spill;    // &spill[global thread], stride spill_stride
+    lds_u32* lds;       // &lds[threadIdx.x], stride kBlock
+    glb_u32* spill;     // &spill[global thread], stride spill_stride
     uint32_t spill_stride;
     uint32_t cap;       // total entries available
+    // Address-space-qualified pointers keep LDS and global accesses as ds_* and
+    // global_* instructions instead of one merged flat access.
+    __device__ Stack(uint32_t* lds_lane, uint32_t* spill_lane, uint32_t stride, uint32_t capacity)
+        : lds((lds_u32*)lds_lane), spill((glb_u32*)spill_lane), spill_stride(stride), cap(capacity) {}
     __device__ void put(int i, uint32_t v) {
         if (i < kLdsStack)
             lds[i * kBlock] = v;
@@ -54,7 +61,8 @@ struct Stack {
             spill[(size_t)(i - kLdsStack) * spill_stride] = v;
     }
     __device__ uint32_t get(int i) const {
-        return (i < kLdsStack) ? lds[i * kBlock] : spill[(size_t)(i - kLdsStack) * spill_stride];
+        if (i < kLdsStack) return lds[i * kBlock];
+        return spill[(size_t)(i - kLdsStack) * spill_stride];
     }
 };
 
@@ -279,22 +287,16 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                 bmn = rmn;
                 bmx = rmx;
             } else {
+                // whole 64-B record (cache-resident), then select the side in registers
                 const uint32_t node = (L.cur & ~kLeafBit) >> 1;
                 const float4* np = sc.nodes + 4 * (size_t)node;
+                const float4 a = np[0], b = np[1], c = np[2];
                 const uint4 r = reinterpret_cast<const uint4*>(np)[3];
-                if (L.cur & 1u) {
-                    const float4 b = np[1], c = np[2];
-                    bmn = V3{b.z, b.w, c.x};
-                    bmx = V3{c.y, c.z, c.w};
-                    first = r.z;
-                    cnt = r.w;
-                } else {
-                    const float4 a = np[0], b = np[1];
-                    bmn = V3{a.x, a.y, a.z};
-                    bmx = V3{a.w, b.x, b.y};
-                    first = r.x;
-                    cnt = r.y;
-                }
+                const bool right = (L.cur & 1u) != 0;
+                bmn = right ? V3{b.z, b.w, c.x} : V3{a.x, a.y, a.z};
+                bmx = right ? V3{c.y, c.z, c.w} : V3{a.w, b.x, b.y};
+                first = right ? r.z : r.x;
+                cnt = right ? r.w : r.y;
             }
             uint32_t dummy_t = 0, dummy_i = 0;
             leaf_test<false, false>(sc, L.o, L.d, first, cnt, bmn, bmx, L.h, dummy_t, dummy_i);

@@ -92,22 +92,25 @@ SPT_HD float box_area(V3 mn, V3 mx) {
 // (IEEE divide). Returns the reference's boolean; when true, [tn, tf] is the
 // parametric interval of the ray's LINE inside the box (no t >= 0 clamp).
 // NaN from 0 * inf falls through the comparisons exactly as on the host.
+// The early exits of the reference are evaluated without branching: every
+// value is computed and the two pass conditions are combined at the end,
+// which gives the same boolean (nothing computed has side effects) and lets a
+// GPU issue all box loads before the first comparison.
 SPT_HD bool slab(V3 o, V3 inv, V3 mn, V3 mx, float& tn, float& tf) {
-    float tmin, tmax, tymin, tymax, tzmin, tzmax;
     const bool px = inv.x >= 0.0f, py = inv.y >= 0.0f, pz = inv.z >= 0.0f;
-    tmin = ((px ? mn.x : mx.x) - o.x) * inv.x;
-    tmax = ((px ? mx.x : mn.x) - o.x) * inv.x;
-    tymin = ((py ? mn.y : mx.y) - o.y) * inv.y;
-    tymax = ((py ? mx.y : mn.y) - o.y) * inv.y;
-    if ((tmin > tymax) || (tymin > tmax)) return false;
+    float tmin = ((px ? mn.x : mx.x) - o.x) * inv.x;
+    float tmax = ((px ? mx.x : mn.x) - o.x) * inv.x;
+    const float tymin = ((py ? mn.y : mx.y) - o.y) * inv.y;
+    const float tymax = ((py ? mx.y : mn.y) - o.y) * inv.y;
+    const float tzmin = ((pz ? mn.z : mx.z) - o.z) * inv.z;
+    const float tzmax = ((pz ? mx.z : mn.z) - o.z) * inv.z;
+    const bool pass_xy = !((tmin > tymax) || (tymin > tmax));
     tmin = (tymin > tmin) ? tymin : tmin;
     tmax = (tymax < tmax) ? tymax : tmax;
-    tzmin = ((pz ? mn.z : mx.z) - o.z) * inv.z;
-    tzmax = ((pz ? mx.z : mn.z) - o.z) * inv.z;
-    if ((tmin > tzmax) || (tzmin > tmax)) return false;
+    const bool pass_z = !((tmin > tzmax) || (tzmin > tmax));
     tn = (tzmin > tmin) ? tzmin : tmin;
     tf = (tzmax < tmax) ? tzmax : tmax;
-    return true;
+    return pass_xy & pass_z;
 }
 
 // AABB::isPointInside, acceleration/aabb.cpp:101-105
@@ -118,20 +121,23 @@ SPT_HD bool point_inside(V3 p, V3 mn, V3 mx) {
 }
 
 // Moller-Trumbore, non-culling branch, primitives/triangle.cpp:22-65 with the
-// edges precomputed exactly as the reference computes them per test.
+// edges precomputed exactly as the reference computes them per test. Every
+// quantity is computed and the reference's rejections (|det| < 1e-6, u and v
+// outside the triangle) are combined at the end: the same acceptance and the
+// same t, u, v, without data-dependent branches.
 SPT_HD bool tri_test(V3 o, V3 d, V3 v0, V3 e1, V3 e2, float& t, float& u, float& v) {
-    V3 pvec = cross(d, e2);
-    float det = dot(e1, pvec);
-    if ((det > -1e-6f) && (det < 1e-6f)) return false;
-    float inv_det = 1.0f / det;
-    V3 tvec = o - v0;
+    const V3 pvec = cross(d, e2);
+    const float det = dot(e1, pvec);
+    const bool det_ok = !((det > -1e-6f) && (det < 1e-6f));
+    const float inv_det = 1.0f / det;
+    const V3 tvec = o - v0;
     u = dot(tvec, pvec) * inv_det;
-    if ((u < 0.0f) || (u > 1.0f)) return false;
-    V3 qvec = cross(tvec, e1);
+    const bool u_ok = !((u < 0.0f) || (u > 1.0f));
+    const V3 qvec = cross(tvec, e1);
     v = dot(d, qvec) * inv_det;
-    if ((v < 0.0f) || ((u + v) > 1.0f)) return false;
+    const bool v_ok = !((v < 0.0f) || ((u + v) > 1.0f));
     t = dot(e2, qvec) * inv_det;
-    return true;
+    return det_ok & u_ok & v_ok;
 }
 
 // Counter-based draws (definition in include/spt.h).

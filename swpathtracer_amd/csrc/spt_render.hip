@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, PathBuf A, Work
         atomicAdd(reinterpret_cast<unsigned long long*>(w.cnt + 10), (unsigned long long)n);
         atomicAdd(reinterpret_cast<unsigned long long*>(w.cnt + 12), 1ull);
     }
-    Stack st{lds + threadIdx.x, spill + gtid, gthreads, a.sc.stack_cap};
+    Stack st(lds + threadIdx.x, spill + gtid, gthreads, a.sc.stack_cap);
     bool overflow = false;
     unsigned long long my_nodes = 0, my_tris = 0;
     if (FETCH) {

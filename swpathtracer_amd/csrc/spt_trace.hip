@@ -15,7 +15,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, const float4* __r
     __shared__ uint32_t lds[kLdsStack * kBlock];
     const uint32_t gthreads = gridDim.x * kBlock;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    Stack st{lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap};
+    Stack st(lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap);
     bool overflow = false;
     for (uint64_t i = gtid; i < n; i += gthreads) {
         const float4 ro = rays[2 * i], rd = rays[2 * i + 1];
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_p(DevScene sc, const float4* _
     __shared__ uint32_t lds[kLdsStack * kBlock];
     const uint32_t gthreads = gridDim.x * kBlock;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    Stack st{lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap};
+    Stack st(lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap);
     bool overflow = false;
     trace_persistent<FETCH>(
         sc, n, flags + 8, st, overflow,
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_raycast(DevScene sc, DevCamera cam, 
     __shared__ uint32_t lds[kLdsStack * kBlock];
     const uint32_t gthreads = gridDim.x * kBlock;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    Stack st{lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap};
+    Stack st(lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap);
     bool overflow = false;
     const uint64_t n = (uint64_t)cam.width * cam.height;
     for (uint64_t i = gtid; i < n; i += gthreads) {
