@@ -312,6 +312,65 @@ spt_status spt_render_get_stats(spt_ctx* ctx, spt_render_stats* out);
 spt_status spt_event_record(spt_ctx* ctx, int slot);
 spt_status spt_event_elapsed(spt_ctx* ctx, int a, int b, float* ms);
 
+/* ------------------------------------------------------------------------- */
+/* Learned path guide (no reference counterpart)                             */
+/* ------------------------------------------------------------------------- */
+
+/* A small MLP maps the encoded hit (position in the scene box, shading normal,
+ * outgoing direction; 32 features) through three 64-unit ReLU layers to an
+ * 8-lobe von Mises-Fisher mixture (40 outputs: per lobe mu_xyz, log kappa,
+ * logit). BF16 weights with FP32 accumulation (MFMA), FP32 Adam, loss scaling.
+ * Rendering with SPT_RENDER_GUIDED samples single-layer Lambertian and GGX
+ * vertices from the one-sample mixture bsdf_fraction * BSDF + (1 -
+ * bsdf_fraction) * guide, so the estimator stays unbiased; other vertices use
+ * the reference's BSDF sampling. SPT_RENDER_GUIDE_TRAIN additionally records
+ * training samples (position, normal, wo, wi, weight = incident radiance x
+ * f cos / pdf, luminance) for spt_guide_train_collected(). */
+#define SPT_GUIDE_PARAMS 13552
+#define SPT_GUIDE_RECORD_FLOATS 24
+enum { SPT_RENDER_GUIDE_TRAIN = 1u << 7 };
+
+typedef struct spt_guide_config {
+    uint64_t seed;         /* parameter initialisation */
+    float learning_rate;   /* Adam step (default 5e-3) */
+    float loss_scale;      /* initial loss scale (default 65536; halved on overflow) */
+    float bsdf_fraction;   /* one-sample mixture weight of BSDF sampling (default 0.5) */
+    uint32_t batch;        /* training records per step / record capacity (default 262144) */
+} spt_guide_config;
+
+typedef struct spt_guide_stats {
+    uint64_t steps;
+    uint64_t skipped_steps;
+    uint64_t records;      /* records collected since the last training step */
+    float last_loss;
+    float loss_scale;
+    double train_ms;       /* host wall time spent in training steps */
+} spt_guide_stats;
+
+/* (Re)initialises the guide (cfg NULL: defaults). */
+spt_status spt_guide_init(spt_ctx* ctx, const spt_guide_config* cfg);
+/* Host copies of the FP32 master parameters (SPT_GUIDE_PARAMS floats):
+ * W1 64x32 | b1 64 | W2 64x64 | b2 64 | W3 64x64 | b3 64 | W4 48x64 | b4 48
+ * (row-major out x in; rows 40..47 of W4 and b4 are padding, kept zero).
+ * Setting parameters resets the Adam moments. */
+spt_status spt_guide_set_params(spt_ctx* ctx, const float* theta);
+spt_status spt_guide_get_params(spt_ctx* ctx, float* theta);
+/* Raw network outputs (n x 40) for queries d_queries (n x 9: position,
+ * normal, outgoing direction; world space). Needs an uploaded scene (its box
+ * normalises positions). */
+spt_status spt_guide_eval(spt_ctx* ctx, const float* d_queries, uint64_t n, float* d_out);
+/* One Adam step on n records (n x SPT_GUIDE_RECORD_FLOATS: position 0-2,
+ * normal 3-5, wo 6-8, wi 9-11, weight 12). loss may be NULL. */
+spt_status spt_guide_train(spt_ctx* ctx, const float* d_records, uint32_t n, float* loss);
+/* The (unscaled) gradient the step would apply, without applying it (host
+ * grad: SPT_GUIDE_PARAMS floats). */
+spt_status spt_guide_gradient(spt_ctx* ctx, const float* d_records, uint32_t n, float* grad, float* loss);
+/* One Adam step on the records collected by the last SPT_RENDER_GUIDE_TRAIN render. */
+spt_status spt_guide_train_collected(spt_ctx* ctx, float* loss);
+spt_status spt_guide_get_stats(spt_ctx* ctx, spt_guide_stats* out);
+/* Device pointer and count of the collected records. */
+spt_status spt_guide_records(spt_ctx* ctx, float** d_records, uint32_t* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,9 @@ SPT_RENDER_NO_SORT = 1 << 3
 SPT_RENDER_GUIDED = 1 << 4
 SPT_RENDER_FETCH_STATS = 1 << 5
 SPT_RENDER_TIME_KERNELS = 1 << 6
+SPT_RENDER_GUIDE_TRAIN = 1 << 7
+SPT_GUIDE_PARAMS = 13552
+SPT_GUIDE_RECORD_FLOATS = 24
 SPT_TRACE_CLOSEST = 0
 SPT_TRACE_COUNT = 1
 SPT_TRACE_UNPRUNED = 2
@@ -90,6 +93,16 @@ class RenderStats(C.Structure):
                 ("extend_launches", C.c_uint64), ("node_fetches", C.c_uint64), ("tri_fetches", C.c_uint64)]
 
 
+class GuideConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("learning_rate", C.c_float), ("loss_scale", C.c_float),
+                ("bsdf_fraction", C.c_float), ("batch", C.c_uint32)]
+
+
+class GuideStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("skipped_steps", C.c_uint64), ("records", C.c_uint64),
+                ("last_loss", C.c_float), ("loss_scale", C.c_float), ("train_ms", C.c_double)]
+
+
 assert C.sizeof(Layer) == 64 and C.sizeof(Material) == 272
 
 # name -> (restype, argtypes); every entry point declared in include/spt.h
@@ -122,6 +135,15 @@ PROTOTYPES = {
     "spt_render_get_stats": (C.c_int32, [C.c_void_p, C.POINTER(RenderStats)]),
     "spt_event_record": (C.c_int32, [C.c_void_p, C.c_int]),
     "spt_event_elapsed": (C.c_int32, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "spt_guide_init": (C.c_int32, [C.c_void_p, C.POINTER(GuideConfig)]),
+    "spt_guide_set_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "spt_guide_get_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "spt_guide_eval": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "spt_guide_train": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
+    "spt_guide_gradient": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_float)]),
+    "spt_guide_train_collected": (C.c_int32, [C.c_void_p, C.POINTER(C.c_float)]),
+    "spt_guide_get_stats": (C.c_int32, [C.c_void_p, C.POINTER(GuideStats)]),
+    "spt_guide_records": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
 }
 
 _lib = None

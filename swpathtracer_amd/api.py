@@ -338,13 +338,74 @@ class Device:
     def render(self, cam: L.Camera, params: L.RenderParams, d_rgb32=None, d_rgb64=None, d_path=None):
         self._check(self.lib.spt_render(self.h, C.byref(cam), C.byref(params), d_rgb32, d_rgb64, d_path))
 
+    # learned guide -----------------------------------------------------
+    def guide_init(self, seed: int = 1, lr: float = 5e-3, loss_scale: float = 65536.0, bsdf_fraction: float = 0.5,
+                   batch: int = 262144):
+        cfg = L.GuideConfig(seed=seed, learning_rate=lr, loss_scale=loss_scale, bsdf_fraction=bsdf_fraction,
+                            batch=batch)
+        self._check(self.lib.spt_guide_init(self.h, C.byref(cfg)))
+
+    def guide_set_params(self, theta: np.ndarray):
+        t = np.ascontiguousarray(theta, np.float32)
+        assert t.size == L.SPT_GUIDE_PARAMS
+        self._check(self.lib.spt_guide_set_params(self.h, t.ctypes.data))
+
+    def guide_get_params(self) -> np.ndarray:
+        t = np.zeros(L.SPT_GUIDE_PARAMS, np.float32)
+        self._check(self.lib.spt_guide_get_params(self.h, t.ctypes.data))
+        return t
+
+    def guide_eval(self, queries: np.ndarray) -> np.ndarray:
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        dq = self.to_device(q)
+        do = self.alloc(q.shape[0] * 40 * 4)
+        self._check(self.lib.spt_guide_eval(self.h, dq.ptr, q.shape[0], do.ptr))
+        self.sync()
+        return do.download(np.float32, (q.shape[0], 40))
+
+    def _records(self, records: np.ndarray) -> DeviceArray:
+        r = np.ascontiguousarray(records, np.float32).reshape(-1, L.SPT_GUIDE_RECORD_FLOATS)
+        return self.to_device(r), r.shape[0]
+
+    def guide_gradient(self, records: np.ndarray):
+        dr, n = self._records(records)
+        g = np.zeros(L.SPT_GUIDE_PARAMS, np.float32)
+        loss = C.c_float()
+        self._check(self.lib.spt_guide_gradient(self.h, dr.ptr, n, g.ctypes.data, C.byref(loss)))
+        return g, loss.value
+
+    def guide_train(self, records: np.ndarray) -> float:
+        dr, n = self._records(records)
+        loss = C.c_float()
+        self._check(self.lib.spt_guide_train(self.h, dr.ptr, n, C.byref(loss)))
+        return loss.value
+
+    def guide_train_collected(self) -> float:
+        loss = C.c_float()
+        self._check(self.lib.spt_guide_train_collected(self.h, C.byref(loss)))
+        return loss.value
+
+    def guide_stats(self) -> dict:
+        st = L.GuideStats()
+        self._check(self.lib.spt_guide_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in L.GuideStats._fields_}
+
+    def guide_records(self) -> np.ndarray:
+        p, n = C.c_void_p(), C.c_uint32()
+        self._check(self.lib.spt_guide_records(self.h, C.byref(p), C.byref(n)))
+        out = np.empty((n.value, L.SPT_GUIDE_RECORD_FLOATS), np.float32)
+        if n.value:
+            self._check(self.lib.spt_memcpy_d2h(self.h, out.ctypes.data, p.value, out.nbytes))
+        return out
+
     def render_stats(self) -> dict:
         st = L.RenderStats()
         self._check(self.lib.spt_render_get_stats(self.h, C.byref(st)))
         return {f: getattr(st, f) for f, _ in L.RenderStats._fields_}
 
     def render_image(self, cam: L.Camera, integ: Integrator, film64: bool = False, per_path: bool = False,
-                     spp_begin: int = 0, spp_count: int = 0, viewport=None, stream: Optional[np.ndarray] = None):
+                     spp_begin: int = 0, spp_count: int = 0, viewport=None, stream: Optional[np.ndarray] = None,
+                     extra_flags: int = 0):
         """Renders and returns (image (h, w, 3) as the reference's pixel values,
         per-path radiance or None). Host convenience for tests and tools."""
         p = integ.params()
@@ -359,7 +420,7 @@ class Device:
         npix = cam.width * cam.height
         film = self.alloc(npix * 3 * (8 if film64 else 4))
         film.zero()
-        p.flags = L.SPT_RENDER_FILM64 if film64 else L.SPT_RENDER_FILM32
+        p.flags = (L.SPT_RENDER_FILM64 if film64 else L.SPT_RENDER_FILM32) | extra_flags
         dpath = None
         if per_path:
             dpath = self.alloc(vw * vh * cnt * 12)
@@ -367,6 +428,8 @@ class Device:
         dstream = None
         if stream is not None:
             stream = np.ascontiguousarray(stream, np.float32)
+            if stream.shape[0] < vw * vh * cnt:
+                raise ValueError("replay stream needs one row per path")
             dstream = self.to_device(stream)
             p.rng = L.SPT_RNG_STREAM
             p.stream_stride = stream.shape[1]

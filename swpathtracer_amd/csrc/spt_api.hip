@@ -151,6 +151,10 @@ void spt_destroy(spt_ctx* ctx) {
     if (ctx->d_spill) (void)hipFree(ctx->d_spill);
     if (ctx->d_flags) (void)hipFree(ctx->d_flags);
     if (ctx->d_paths) (void)hipFree(ctx->d_paths);
+    if (ctx->guide.d_theta) (void)hipFree(ctx->guide.d_theta);
+    if (ctx->guide.d_records) (void)hipFree(ctx->guide.d_records);
+    if (ctx->guide.d_gout) (void)hipFree(ctx->guide.d_gout);
+    if (ctx->d_guidable) (void)hipFree(ctx->d_guidable);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->tev) (void)hipEventDestroy(e);
@@ -321,6 +325,19 @@ spt_status spt_set_materials(spt_ctx* ctx, const spt_material* mats, uint32_t n_
     if (s != SPT_OK) return s;
     ctx->n_mats = n_mats;
     ctx->h_mats.assign(mats, mats + n_mats);
+    // vertices the learned guide may sample: one layer with an evaluable density
+    std::vector<uint8_t> g(n_mats, 0);
+    for (uint32_t i = 0; i < n_mats; ++i) {
+        const spt_layer& l = mats[i].layers[0];
+        g[i] = (mats[i].n_layers == 1 &&
+                (l.type == SPT_BSDF_LAMBERTIAN || (l.type == SPT_BSDF_GGX_CONDUCTOR && l.sampler == SPT_SAMPLER_IMPORTANCE)))
+                   ? 1
+                   : 0;
+    }
+    if (ctx->d_guidable) (void)hipFree(ctx->d_guidable);
+    ctx->d_guidable = nullptr;
+    s = upload(ctx, &ctx->d_guidable, g.data(), g.size(), "guidable flags");
+    if (s != SPT_OK) return s;
     return SPT_OK;
 }
 

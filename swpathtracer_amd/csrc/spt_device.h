@@ -359,6 +359,32 @@ __device__ inline uint32_t wave_add(uint32_t* counters, uint32_t key) {
     return slot;
 }
 
+// Scene box used by the guide's input encoding.
+struct GuideBox {
+    float bmin[3], bext[3];
+};
+
+// Learned guide state of a context (spt_guide.hip).
+struct GuideState {
+    bool ready = false;
+    float* d_theta = nullptr;  // fp32 master parameters, then m, v, grad, acc (64 floats)
+    float* d_m = nullptr;
+    float* d_v = nullptr;
+    float* d_grad = nullptr;
+    float* d_acc = nullptr;
+    unsigned short* d_wbf = nullptr;  // bf16 copy of the parameters
+    float* d_records = nullptr;
+    uint32_t records_cap = 0;
+    uint64_t collected = 0;
+    float lr = 5e-3f, loss_scale = 65536.0f, bsdf_fraction = 0.5f;
+    uint32_t batch = 262144;
+    uint64_t step = 0, skipped = 0;
+    float last_loss = 0.0f;
+    double train_ms = 0.0;
+    float* d_gout = nullptr;  // per-path raw outputs during a guided render
+    size_t gout_cap = 0;
+};
+
 }  // namespace spt
 
 // The context: one device, one stream, the uploaded scene and scratch buffers.
@@ -379,6 +405,7 @@ struct spt_ctx {
     uint32_t max_mat = 0;  // largest material id referenced by a triangle
     bool has_scene = false;
     std::vector<spt_material> h_mats;
+    uint8_t* d_guidable = nullptr;  // per material: 1 = single Lambertian / GGX layer (guide-sampled)
     // traversal stack spill buffer
     uint32_t* d_spill = nullptr;
     size_t spill_words = 0;
@@ -393,6 +420,7 @@ struct spt_ctx {
     void* d_paths = nullptr;
     size_t paths_bytes = 0;
     spt_render_stats stats{};
+    spt::GuideState guide;
 
     spt::DevScene scene() const {
         spt::DevScene s;
@@ -423,6 +451,8 @@ spt_status launch_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_h
                         uint32_t* d_tests, uint32_t* d_isects);
 spt_status launch_raycast(spt_ctx* ctx, const spt_camera* cam, uint32_t kind, void* d_out);
 spt_status launch_camera_rays(spt_ctx* ctx, const spt_camera* cam, spt_ray32* d_rays);
+GuideBox guide_box(const spt_ctx* ctx);
+spt_status guide_eval_launch(spt_ctx* ctx, const float* d_q, uint32_t stride, uint32_t n, float* d_out);
 spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* p, float* d_rgb32,
                          double* d_rgb64, float* d_path_rgb);
 }  // namespace spt

@@ -1,0 +1,309 @@
+// spt_guide.h - learned path guide on the device: input encoding, a 32-64-64-
+// 64-40 MLP evaluated 16 rows per wave with bf16 MFMA (v_mfma_f32_16x16x16_bf16,
+// fp32 accumulation), the 8-lobe von Mises-Fisher mixture it outputs (density,
+// sampling, KL-loss gradient) and the fused backward pass.
+//
+// Parameter vector (fp32 master, 13,552 floats): W1 64x32 | b1 64 | W2 64x64 |
+// b2 64 | W3 64x64 | b3 64 | W4 48x64 (rows 40..47 zero) | b4 48. Weights are
+// row-major (out, in). Methods: vMF sampling after W. Jakob (2012); KL training
+// of a sampling density on Monte Carlo targets after Mueller et al. (2019).
+#pragma once
+
+#include "spt_device.h"
+
+namespace spt {
+namespace guide {
+
+constexpr int kEnc = 32, kHid = 64, kOut = 40, kOutP = 48, kLobes = 8;
+constexpr int kW1 = 0, kB1 = kW1 + kHid * kEnc, kW2 = kB1 + kHid, kB2 = kW2 + kHid * kHid, kW3 = kB2 + kHid,
+              kB3 = kW3 + kHid * kHid, kW4 = kB3 + kHid, kB4 = kW4 + kOutP * kHid, kParams = kB4 + kOutP;
+static_assert(kParams == 13552, "parameter layout");
+constexpr int kRecord = 24;  // floats per training record: p n wo wi W g rad_after thr_next pad
+constexpr float kKappaMin = 1e-2f, kKappaMax = 1e4f;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// round-to-nearest-even float -> bfloat16 bits, and back
+__device__ inline unsigned short f2bf(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short b) { return __uint_as_float((uint32_t)b << 16); }
+
+// Input encoding (32): x = (p - bmin) / bext clamped to [0,1]; sin/cos of
+// 2 pi 2^k x_c for k = 0..2; n; wo; one-blob (4 Gaussian bins, sigma 1/4) of
+// (n.wo + 1)/2; constant 1.
+__device__ inline void encode(V3 p, V3 n, V3 wo, const float* bmin, const float* bext, float f[kEnc]) {
+    const float x[3] = {fminf(fmaxf((p.x - bmin[0]) / bext[0], 0.0f), 1.0f),
+                        fminf(fmaxf((p.y - bmin[1]) / bext[1], 0.0f), 1.0f),
+                        fminf(fmaxf((p.z - bmin[2]) / bext[2], 0.0f), 1.0f)};
+    f[0] = x[0];
+    f[1] = x[1];
+    f[2] = x[2];
+    int o = 3;
+    for (int k = 0; k < 3; ++k) {
+        for (int c = 0; c < 3; ++c) {
+            const float a = 2.0f * kPi * (float)(1 << k) * x[c];
+            f[o++] = sinf(a);
+            f[o++] = cosf(a);
+        }
+    }
+    f[21] = n.x;
+    f[22] = n.y;
+    f[23] = n.z;
+    f[24] = wo.x;
+    f[25] = wo.y;
+    f[26] = wo.z;
+    const float c = 0.5f * (dot(n, wo) + 1.0f);
+    for (int b = 0; b < 4; ++b) {
+        const float dc = c - (b + 0.5f) * 0.25f;
+        f[27 + b] = expf(-dc * dc / (2.0f * 0.0625f));
+    }
+    f[31] = 1.0f;
+}
+
+// Per-wave LDS tiles (bf16 bits). Rows = 16 queries.
+struct WaveTiles {
+    unsigned short x[16 * kEnc];
+    unsigned short h1[16 * kHid];
+    unsigned short h2[16 * kHid];
+    unsigned short h3[16 * kHid];
+    float y[16 * kOutP];
+    unsigned short dy[16 * kOutP];
+    unsigned short da[16 * kHid];
+    unsigned short db[16 * kHid];
+};
+
+// Block-shared weights: bf16 copies of W1..W4 and fp32 biases.
+struct SharedWeights {
+    unsigned short w1[kHid * kEnc];
+    unsigned short w2[kHid * kHid];
+    unsigned short w3[kHid * kHid];
+    unsigned short w4[kOutP * kHid];
+    float b1[kHid], b2[kHid], b3[kHid], b4[kOutP];
+};
+
+__device__ inline void load_weights(SharedWeights& s, const unsigned short* wbf, const float* theta) {
+    for (int i = threadIdx.x; i < kHid * kEnc; i += blockDim.x) s.w1[i] = wbf[kW1 + i];
+    for (int i = threadIdx.x; i < kHid * kHid; i += blockDim.x) {
+        s.w2[i] = wbf[kW2 + i];
+        s.w3[i] = wbf[kW3 + i];
+    }
+    for (int i = threadIdx.x; i < kOutP * kHid; i += blockDim.x) s.w4[i] = wbf[kW4 + i];
+    for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+        s.b1[i] = theta[kB1 + i];
+        s.b2[i] = theta[kB2 + i];
+        s.b3[i] = theta[kB3 + i];
+    }
+    for (int i = threadIdx.x; i < kOutP; i += blockDim.x) s.b4[i] = theta[kB4 + i];
+}
+
+// 4 consecutive bf16 of an LDS row as an MFMA operand
+__device__ inline bf16x4 ld4(const unsigned short* p) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    bf16x4 r;
+    r[0] = (short)(v.x & 0xFFFFu);
+    r[1] = (short)(v.x >> 16);
+    r[2] = (short)(v.y & 0xFFFFu);
+    r[3] = (short)(v.y >> 16);
+    return r;
+}
+// 4 bf16 with a stride
+__device__ inline bf16x4 ld4s(const unsigned short* p, int stride) {
+    bf16x4 r;
+    r[0] = (short)p[0];
+    r[1] = (short)p[stride];
+    r[2] = (short)p[2 * stride];
+    r[3] = (short)p[3 * stride];
+    return r;
+}
+
+// v_mfma_f32_16x16x16_bf16 lane layout (wave64), lane l, r = l % 16, g = l / 16:
+//   A (16 rows x 16 k):  A[r][4g + j]         j = 0..3
+//   B (16 k x 16 cols):  B[4g + j][r]
+//   D (16 x 16, f32):    D[4g + i][r]         i = 0..3
+// Dense layer on one 16-row tile: out[row][c] = act(sum_k in[row][k] W[c][k] + b[c])
+// in: LDS [16][K] bf16; W: LDS [N][K] bf16 (row-major out x in).
+template <int K, int N, bool RELU>
+__device__ inline void dense_fwd(const unsigned short* in, const unsigned short* W, const float* b, unsigned short* out_bf,
+                                 float* out_f) {
+    const int l = __lane_id(), r = l & 15, g = l >> 4;
+#pragma unroll
+    for (int t = 0; t < N / 16; ++t) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < K / 16; ++s) {
+            const bf16x4 a = ld4(in + r * K + 16 * s + 4 * g);
+            const bf16x4 w = ld4(W + (16 * t + r) * K + 16 * s + 4 * g);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc, 0, 0, 0);
+        }
+        const int col = 16 * t + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float v = acc[i] + b[col];
+            if (RELU) v = fmaxf(v, 0.0f);
+            const int row = 4 * g + i;
+            if (out_bf) out_bf[row * N + col] = f2bf(v);
+            if (out_f) out_f[row * N + col] = v;
+        }
+    }
+}
+
+// Full forward of one tile: t.x -> t.h1 -> t.h2 -> t.h3 -> t.y (f32).
+__device__ inline void mlp_tile_fwd(const SharedWeights& s, WaveTiles& t) {
+    dense_fwd<kEnc, kHid, true>(t.x, s.w1, s.b1, t.h1, nullptr);
+    dense_fwd<kHid, kHid, true>(t.h1, s.w2, s.b2, t.h2, nullptr);
+    dense_fwd<kHid, kHid, true>(t.h2, s.w3, s.b3, t.h3, nullptr);
+    dense_fwd<kHid, kOutP, false>(t.h3, s.w4, s.b4, nullptr, t.y);
+}
+
+// Mixture parameters of one raw output row.
+struct Lobe {
+    V3 mu;
+    float kappa, logw, norm, s;
+    bool inside;  // log kappa inside its clamp range (gradient flows)
+};
+
+__device__ inline void mixture_params(const float* y, Lobe L[kLobes]) {
+    float amax = -3.0e38f;
+    for (int i = 0; i < kLobes; ++i) amax = fmaxf(amax, y[5 * i + 4]);
+    float sum = 0.0f;
+    for (int i = 0; i < kLobes; ++i) sum += expf(y[5 * i + 4] - amax);
+    const float lse = amax + logf(sum);
+    for (int i = 0; i < kLobes; ++i) {
+        const V3 raw{y[5 * i], y[5 * i + 1], y[5 * i + 2]};
+        const float nr = sqrtf(dot(raw, raw) + 1e-12f);
+        L[i].mu = raw * (1.0f / nr);
+        L[i].norm = nr;
+        const float s = y[5 * i + 3];
+        const float sc = fminf(fmaxf(s, logf(kKappaMin)), logf(kKappaMax));
+        L[i].inside = (s == sc);
+        L[i].s = sc;
+        L[i].kappa = expf(sc);
+        L[i].logw = y[5 * i + 4] - lse;
+    }
+}
+
+// log of kappa / (2 pi (1 - e^{-2 kappa})) exp(kappa (mu.d - 1))
+__device__ inline float log_vmf(const Lobe& L, V3 d) {
+    return logf(L.kappa) - 1.8378770664093453f - logf(-expm1f(-2.0f * L.kappa)) + L.kappa * (dot(L.mu, d) - 1.0f);
+}
+
+__device__ inline float mixture_pdf(const Lobe L[kLobes], V3 d) {
+    float q = 0.0f;
+    for (int i = 0; i < kLobes; ++i) q += expf(L[i].logw + log_vmf(L[i], d));
+    return q;
+}
+
+// Orthonormal basis around n (Duff et al. 2017).
+__device__ inline void basis(V3 n, V3& t, V3& b) {
+    const float s = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (s + n.z);
+    const float bb = n.x * n.y * a;
+    t = V3{1.0f + s * n.x * n.x * a, s * bb, -s * n.x};
+    b = V3{bb, s + n.y * n.y * a, -n.y};
+}
+
+// Sample the mixture: lobe by weight (u0), then Jakob's stable vMF sampler.
+__device__ inline V3 mixture_sample(const Lobe L[kLobes], float u0, float u1, float u2) {
+    int k = kLobes - 1;
+    float c = 0.0f;
+    for (int i = 0; i < kLobes; ++i) {
+        c += expf(L[i].logw);
+        if (u0 < c) {
+            k = i;
+            break;
+        }
+    }
+    const float kap = L[k].kappa;
+    const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kap)) / kap;
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+    const float ph = 2.0f * kPi * u2;
+    V3 t, b;
+    basis(L[k].mu, t, b);
+    return t * (st * cosf(ph)) + b * (st * sinf(ph)) + L[k].mu * w;
+}
+
+// Gradient of -W log q(d) w.r.t. the 40 raw outputs (dy[40..47] = 0).
+__device__ inline float loss_grad(const float* y, V3 d, float W, float* dy) {
+    Lobe L[kLobes];
+    mixture_params(y, L);
+    float lv[kLobes];
+    float m = -3.0e38f;
+    for (int i = 0; i < kLobes; ++i) {
+        lv[i] = L[i].logw + log_vmf(L[i], d);
+        m = fmaxf(m, lv[i]);
+    }
+    float sum = 0.0f;
+    for (int i = 0; i < kLobes; ++i) sum += expf(lv[i] - m);
+    const float logq = m + logf(sum);
+    for (int i = 0; i < kLobes; ++i) {
+        const float r = expf(lv[i] - logq);
+        const float w = expf(L[i].logw);
+        const float c = dot(L[i].mu, d);
+        dy[5 * i + 4] = -W * (r - w);
+        const float kap = L[i].kappa;
+        const float dlv = 1.0f - 2.0f * kap / expm1f(2.0f * kap) + kap * (c - 1.0f);
+        dy[5 * i + 3] = L[i].inside ? -W * r * dlv : 0.0f;
+        const V3 gmu = d * (-W * r * kap);
+        const V3 proj = (gmu - L[i].mu * dot(gmu, L[i].mu)) * (1.0f / L[i].norm);
+        dy[5 * i + 0] = proj.x;
+        dy[5 * i + 1] = proj.y;
+        dy[5 * i + 2] = proj.z;
+    }
+    for (int i = kOut; i < kOutP; ++i) dy[i] = 0.0f;
+    return -W * logq;
+}
+
+// dW[m][n] += sum_rows dOut[row][m] * In[row][n]  (MFMA over the 16 rows),
+// added into an fp32 LDS accumulator with ds_add_f32.
+template <int M, int N>
+__device__ inline void dense_wgrad(const unsigned short* dout, const unsigned short* in, float* dW) {
+    const int l = __lane_id(), r = l & 15, g = l >> 4;
+#pragma unroll
+    for (int mt = 0; mt < M / 16; ++mt) {
+        // A[i][k] = dout[k][16 mt + i]: rows of the tile are the reduction axis
+        const bf16x4 a = ld4s(dout + (4 * g) * M + 16 * mt + r, M);
+#pragma unroll
+        for (int nt = 0; nt < N / 16; ++nt) {
+            const bf16x4 b = ld4s(in + (4 * g) * N + 16 * nt + r, N);
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) atomicAdd(&dW[(16 * mt + 4 * g + i) * N + 16 * nt + r], acc[i]);
+        }
+    }
+}
+
+// dIn[row][n] = (sum_m dOut[row][m] W[m][n]) * (in[row][n] > 0); bias grad of
+// this layer's input side is accumulated by the caller.
+template <int M, int N>
+__device__ inline void dense_igrad(const unsigned short* dout, const unsigned short* W, const unsigned short* act,
+                                   unsigned short* din, float* dbias) {
+    const int l = __lane_id(), r = l & 15, g = l >> 4;
+#pragma unroll
+    for (int t = 0; t < N / 16; ++t) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < M / 16; ++s) {
+            const bf16x4 a = ld4(dout + r * M + 16 * s + 4 * g);
+            const bf16x4 w = ld4s(W + (16 * s + 4 * g) * N + 16 * t + r, N);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc, 0, 0, 0);
+        }
+        const int col = 16 * t + r;
+        float bsum = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * g + i;
+            const float v = (bf2f(act[row * N + col]) > 0.0f) ? acc[i] : 0.0f;
+            din[row * N + col] = f2bf(v);
+            bsum += v;
+        }
+        if (dbias) atomicAdd(&dbias[col], bsum);
+    }
+}
+
+}  // namespace guide
+}  // namespace spt

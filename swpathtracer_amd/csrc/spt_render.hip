@@ -12,6 +12,7 @@
 #include <chrono>
 
 #include "spt_bsdf.h"
+#include "spt_guide.h"
 
 namespace spt {
 
@@ -31,8 +32,10 @@ struct PathBuf {
     uint4* meta;     // pixel index, draws consumed, depth, unused
     uint64_t* key;   // RNG key (counter mode) or replay-stream row
     uint64_t* local; // path index within this render call (per-path output)
+    uint4* grec;     // guide training records of this path: 3 slots + count
 };
-constexpr size_t kBufBytes = 16 * 5 + 8 * 2;
+constexpr size_t kBufBytes = 16 * 6 + 8 * 2;
+constexpr uint32_t kNoRec = 0xFFFFFFFFu;
 
 // Per-bounce scratch shared by both buffers.
 struct Work {
@@ -61,7 +64,29 @@ struct RenderArgs {
     double* film64;
     float* path_rgb;
     uint32_t keys;  // sort keys (n_mats + 1), 0 = no sort
+    // learned guide
+    uint32_t guided, train;
+    const uint8_t* guidable;  // per material: single Lambertian / GGX layer
+    float bsdf_frac;
+    float* gout;              // per path slot: 40 raw network outputs
+    float* records;           // training records (guide::kRecord floats each)
+    uint32_t* rec_count;
+    uint32_t rec_cap;
 };
+
+__device__ inline float lum(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// Training targets of a finished path: the radiance that arrived along each
+// recorded wi is (L - radiance before it) / throughput after it.
+__device__ inline void finish_records(const RenderArgs& a, uint4 gr, float4 L) {
+    const uint32_t sl[3] = {gr.x, gr.y, gr.z};
+    for (uint32_t k = 0; k < gr.w && k < 3; ++k) {
+        float* r = a.records + (size_t)sl[k] * guide::kRecord;
+        const float lt = lum(V3{r[17], r[18], r[19]});
+        const float li = lum(V3{L.x - r[14], L.y - r[15], L.z - r[16]});
+        r[12] = (lt > 0.0f && li > 0.0f) ? r[13] * li / lt : 0.0f;
+    }
+}
 
 __device__ inline void finish_path(const RenderArgs& a, uint32_t pix, uint64_t local, float4 L) {
     if (a.film32) {
@@ -134,6 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(RenderArgs a, PathBuf A, Wo
     A.meta[s] = make_uint4(pix, r.k, 0u, 0u);
     A.key[s] = key;
     A.local[s] = g;
+    if (a.train) A.grec[s] = make_uint4(kNoRec, kNoRec, kNoRec, 0u);
 }
 
 // Closest hit of every live path of buffer A (count *cntA). Persistent
@@ -278,6 +304,73 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RenderArgs a, Work w, const 
     }
 }
 
+// Guide inference for the live hits of buffer A whose material is guidable,
+// 16 rows per wave in shading order; raw outputs go to gout[slot].
+__global__ __launch_bounds__(kBlock) void k_guide_paths(RenderArgs a, PathBuf A, Work w,
+                                                        const uint32_t* __restrict__ cntA,
+                                                        const float* __restrict__ theta,
+                                                        const unsigned short* __restrict__ wbf, GuideBox box) {
+    using namespace guide;
+    extern __shared__ float dyn[];
+    SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
+    WaveTiles* tiles = reinterpret_cast<WaveTiles*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
+    const uint32_t n = *cntA;
+    if (n == 0) return;
+    load_weights(sw, wbf, theta);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, l = __lane_id();
+    WaveTiles& t = tiles[wave];
+    const uint32_t ntiles = (n + 15) / 16;
+    for (uint32_t base = blockIdx.x * (kBlock / 64); base < ntiles; base += gridDim.x * (kBlock / 64)) {
+        const uint32_t tile = base + wave;
+        bool mine = false;
+        uint32_t slot = 0;
+        if (tile < ntiles && l < 16) {
+            const uint32_t j = tile * 16 + l;
+            float f[kEnc];
+            if (j < n) {
+                slot = a.keys ? w.perm[j] : j;
+                const float4 hv = w.hit[slot];
+                const uint32_t idx = __float_as_uint(hv.y);
+                mine = (idx != kNoHit) && a.guidable[w.skey[slot]];
+                if (mine) {
+                    const float4 o4 = A.o[slot], d4 = A.d[slot];
+                    const V3 d{d4.x, d4.y, d4.z};
+                    Hit h;
+                    h.t = hv.x;
+                    h.idx = idx;
+                    h.u = hv.z;
+                    h.v = hv.w;
+                    const V3 pos = V3{o4.x, o4.y, o4.z} + d * h.t;
+                    encode(pos, shading_normal(a.sc, h), -d, box.bmin, box.bext, f);
+                }
+            }
+            if (!mine)
+                for (int i = 0; i < kEnc; ++i) f[i] = 0.0f;
+            for (int i = 0; i < kEnc; ++i) t.x[l * kEnc + i] = f2bf(f[i]);
+        }
+        const uint64_t any = __ballot(mine);
+        __syncthreads();
+        if (tile < ntiles && any) mlp_tile_fwd(sw, t);
+        __syncthreads();
+        if (tile < ntiles && any) {
+            // lane r (< 16) writes its row's 40 outputs
+            if (mine)
+                for (int c = 0; c < kOut; ++c) a.gout[(size_t)slot * kOut + c] = t.y[l * kOutP + c];
+        }
+        __syncthreads();
+    }
+}
+
+// Density of a single BSDF layer's own sampler (guidable layers only).
+__device__ inline float layer_density(const spt_layer& l, V3 wo, V3 wi) {
+    if (l.type == SPT_BSDF_LAMBERTIAN) {
+        if (wi.y <= 0.0f) return 0.0f;
+        return (l.sampler == SPT_SAMPLER_UNIFORM) ? 1.0f / (2.0f * kPi) : wi.y / kPi;
+    }
+    return ggx_pdf(l, wo, wi);
+}
+
 // Shading of one bounce (integrators/pathtracer.cpp:21-54) for every live
 // path of A, in material order when sorted; survivors are appended to B.
 __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathBuf B, Work w,
@@ -298,6 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
                 rad.y += thr.y * a.bg[1];
                 rad.z += thr.z * a.bg[2];
             }
+            if (a.train) finish_records(a, A.grec[i], rad);
             finish_path(a, meta.x, A.local[i], rad);
             continue;
         }
@@ -310,14 +404,40 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         h.v = hv.w;
         const V3 pos = o + d * h.t;  // triangle.cpp:69
         const V3 nrm = shading_normal(a.sc, h);
-        const spt_material& m = a.sc.mats[w.skey[i]];
+        const uint32_t mid = w.skey[i];
+        const spt_material& m = a.sc.mats[mid];
         const Frame fr = frame_from_up(nrm);          // pathtracer.cpp:28
         const V3 wo = normalize(to_local(fr, -d));    // pathtracer.cpp:30
         const uint64_t key = A.key[i];
         Draws r = make_draws(a, key, meta.y);
         V3 bsdf, refl, wi;
         float pdf;
-        layered_eval(m, wo, bsdf, refl, pdf, wi, r);  // pathtracer.cpp:36
+        const bool guided = a.guided && a.guidable[mid];
+        if (guided) {
+            // one-sample mixture of the layer's sampler and the guide
+            using namespace guide;
+            const spt_layer& l = m.layers[0];
+            Lobe lobes[kLobes];
+            mixture_params(a.gout + (size_t)i * kOut, lobes);
+            if (r.next() < a.bsdf_frac) {
+                wi = layer_sample(l, wo, r).wi;
+            } else {
+                const float u0 = r.next(), u1 = r.next(), u2 = r.next();
+                wi = to_local(fr, mixture_sample(lobes, u0, u1, u2));
+            }
+            pdf = a.bsdf_frac * layer_density(l, wo, wi) +
+                  (1.0f - a.bsdf_frac) * mixture_pdf(lobes, to_world(fr, wi));
+            if (wi.y > 0.0f && pdf > 0.0f) {
+                bsdf = layer_f(l, wo, wi);
+                refl = layer_reflectance(l, wo, wi);
+            } else {
+                bsdf = V3{0.0f, 0.0f, 0.0f};
+                refl = V3{0.0f, 0.0f, 0.0f};
+                pdf = 1.0f;
+            }
+        } else {
+            layered_eval(m, wo, bsdf, refl, pdf, wi, r);  // pathtracer.cpp:36
+        }
         const V3 wwi = to_world(fr, wi);              // pathtracer.cpp:37
         const V3 no = pos + wwi * 1e-3f;              // pathtracer.cpp:38
         rad.x += thr.x * m.emission[0];
@@ -335,11 +455,28 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         const uint32_t nd = depth + 1;
         if (alive && (a.max_depth >= 0) && ((int32_t)nd >= a.max_depth)) alive = false;  // pathtracer.cpp:18
         if (!alive) {
+            if (a.train) finish_records(a, A.grec[i], rad);
             finish_path(a, meta.x, A.local[i], rad);
             continue;
         }
         // pathtracer.cpp:52-54, forwards: throughput *= f * cos / pdf
         const V3 wgt = (bsdf * wi.y) / pdf;
+        uint4 gr = a.train ? A.grec[i] : make_uint4(0u, 0u, 0u, 0u);
+        if (a.train && guided && gr.w < 3u && wi.y > 0.0f && bsdf.x + bsdf.y + bsdf.z > 0.0f) {
+            const uint32_t rs = atomicAdd(a.rec_count, 1u);
+            if (rs < a.rec_cap) {
+                float* rec = a.records + (size_t)rs * guide::kRecord;
+                const V3 wdir = to_world(fr, wi);
+                const float vals[20] = {pos.x, pos.y, pos.z, nrm.x, nrm.y, nrm.z, -d.x, -d.y, -d.z, wdir.x, wdir.y,
+                                        wdir.z, 0.0f, lum(bsdf) * wi.y / pdf, rad.x, rad.y, rad.z,
+                                        thr.x * wgt.x, thr.y * wgt.y, thr.z * wgt.z};
+                for (int k = 0; k < 20; ++k) rec[k] = vals[k];
+                if (gr.w == 0) gr.x = rs;
+                else if (gr.w == 1) gr.y = rs;
+                else gr.z = rs;
+                ++gr.w;
+            }
+        }
         const uint32_t slot = atomicAdd(cntB, 1u);
         B.o[slot] = make_float4(no.x, no.y, no.z, 0.0f);
         B.d[slot] = make_float4(wwi.x, wwi.y, wwi.z, 0.0f);
@@ -348,6 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         B.meta[slot] = make_uint4(meta.x, r.k, nd, 0u);
         B.key[slot] = key;
         B.local[slot] = A.local[i];
+        if (a.train) B.grec[slot] = gr;
     }
 }
 
@@ -361,6 +499,10 @@ __global__ void k_flip(uint32_t* cntA, uint32_t* fetch) {
 }
 
 uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+size_t guide_paths_lds_bytes() {
+    return sizeof(guide::SharedWeights) + sizeof(guide::WaveTiles) * (kBlock / 64);
+}
 
 }  // namespace
 
@@ -411,7 +553,15 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     if ((prm->flags & SPT_RENDER_FILM64) && !d_rgb64) return set_error(ctx, SPT_ERR_INVALID, "FILM64 without buffer");
     if ((prm->flags & SPT_RENDER_PER_PATH) && !d_path_rgb)
         return set_error(ctx, SPT_ERR_INVALID, "PER_PATH without buffer");
-    if (prm->flags & SPT_RENDER_GUIDED) return set_error(ctx, SPT_ERR_UNSUPPORTED, "guided rendering not enabled");
+    a.guided = (prm->flags & SPT_RENDER_GUIDED) ? 1u : 0u;
+    a.train = (a.guided && (prm->flags & SPT_RENDER_GUIDE_TRAIN)) ? 1u : 0u;
+    if ((prm->flags & SPT_RENDER_GUIDE_TRAIN) && !a.guided)
+        return set_error(ctx, SPT_ERR_INVALID, "GUIDE_TRAIN needs GUIDED");
+    if (a.guided && !ctx->guide.ready) return set_error(ctx, SPT_ERR_STATE, "guided rendering needs spt_guide_init");
+    a.guidable = ctx->d_guidable;
+    a.bsdf_frac = ctx->guide.bsdf_fraction;
+    a.records = ctx->guide.d_records;
+    a.rec_cap = ctx->guide.records_cap;
     if (prm->rng == SPT_RNG_STREAM) {
         if (!prm->d_stream || prm->stream_stride == 0)
             return set_error(ctx, SPT_ERR_INVALID, "replay stream missing");
@@ -452,6 +602,7 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
         b.meta = (uint4*)take(cap * 16);
         b.key = (uint64_t*)take(cap * 8);
         b.local = (uint64_t*)take(cap * 8);
+        b.grec = (uint4*)take(cap * 16);
     }
     Work w;
     w.hit = (float4*)take(cap * 16);
@@ -461,6 +612,20 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     w.cnt = (uint32_t*)take(256);
     if ((size_t)(base - reinterpret_cast<char*>(ctx->d_paths)) > ctx->paths_bytes)
         return set_error(ctx, SPT_ERR_NOMEM, "path state layout exceeds its allocation");
+    a.rec_count = w.cnt + 16;
+    if (a.guided) {
+        const size_t need = (size_t)cap * guide::kOut * sizeof(float);
+        if (need > ctx->guide.gout_cap) {
+            (void)hipStreamSynchronize(ctx->stream);
+            if (ctx->guide.d_gout) (void)hipFree(ctx->guide.d_gout);
+            ctx->guide.d_gout = nullptr;
+            ctx->guide.gout_cap = 0;
+            spt_status s = check_hip(ctx, hipMalloc(&ctx->guide.d_gout, need), "guide outputs");
+            if (s != SPT_OK) return s;
+            ctx->guide.gout_cap = need;
+        }
+        a.gout = ctx->guide.d_gout;
+    }
 
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_extend<true>),
@@ -477,6 +642,23 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     const bool fetch_stats = (prm->flags & SPT_RENDER_FETCH_STATS) != 0;
     s = check_hip(ctx, hipMemsetAsync(w.cnt, 0, 256, st), "counters");
     if (s != SPT_OK) return s;
+    if (a.train) {  // records continue after those collected since the last training step
+        s = check_hip(ctx, hipMemsetD32Async((hipDeviceptr_t)(w.cnt + 16), (int)ctx->guide.collected, 1, st),
+                      "record counter");
+        if (s != SPT_OK) return s;
+    }
+    if (a.guided) {
+        static bool lds_ok = false;
+        if (!lds_ok) {
+            s = check_hip(ctx,
+                          hipFuncSetAttribute(reinterpret_cast<const void*>(&k_guide_paths),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)guide_paths_lds_bytes()),
+                          "guide LDS size");
+            if (s != SPT_OK) return s;
+            lds_ok = true;
+        }
+    }
     if (fetch_stats) {
         s = check_hip(ctx, hipMemsetAsync(ctx->d_flags + 4, 0, 16, st), "fetch counters");
         if (s != SPT_OK) return s;
@@ -521,6 +703,10 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
                 hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, a, w);
                 hipLaunchKernelGGL(k_scatter, dim3(kSortBlocks), dim3(kBlock), 0, st, a, w, cntA);
             }
+            if (a.guided)
+                hipLaunchKernelGGL(k_guide_paths, dim3((uint32_t)ctx->num_cus * 2), dim3(kBlock),
+                                   guide_paths_lds_bytes(), st, a, A, w, cntA, ctx->guide.d_theta, ctx->guide.d_wbf,
+                                   guide_box(ctx));
             hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kBlock), 0, st, a, A, B, w, cntA, cntB);
             hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, st, cntA, w.cnt + 8);
             s = check_hip(ctx, hipGetLastError(), "bounce launch");
@@ -542,6 +728,7 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "render");
     if (s != SPT_OK) return s;
     stats.stream_overflows = cnt[2];
+    if (a.train) ctx->guide.collected = std::min<uint64_t>(cnt[16], a.rec_cap);
     stats.rays = (uint64_t)cnt[10] | ((uint64_t)cnt[11] << 32);
     stats.bounces = (uint64_t)cnt[12] | ((uint64_t)cnt[13] << 32);
     if (timed) {
