@@ -444,6 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         rad.y += thr.y * m.emission[1];
         rad.z += thr.z * m.emission[2];
         const float kk = gmax(gmax(refl.x, refl.y), refl.z);  // pathtracer.cpp:41
+        const float rec_w = lum(bsdf) * wi.y / pdf;           // training weight, before roulette
         bool alive = true;
         if ((a.rr_depth >= 0) && ((int32_t)depth >= a.rr_depth)) {  // pathtracer.cpp:43-49
             if ((r.next() < kk) && (depth < 1000u))
@@ -468,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
                 float* rec = a.records + (size_t)rs * guide::kRecord;
                 const V3 wdir = to_world(fr, wi);
                 const float vals[20] = {pos.x, pos.y, pos.z, nrm.x, nrm.y, nrm.z, -d.x, -d.y, -d.z, wdir.x, wdir.y,
-                                        wdir.z, 0.0f, lum(bsdf) * wi.y / pdf, rad.x, rad.y, rad.z,
+                                        wdir.z, 0.0f, rec_w, rad.x, rad.y, rad.z,
                                         thr.x * wgt.x, thr.y * wgt.y, thr.z * wgt.z};
                 for (int k = 0; k < 20; ++k) rec[k] = vals[k];
                 if (gr.w == 0) gr.x = rs;

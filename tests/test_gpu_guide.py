@@ -1,0 +1,243 @@
+"""The learned guide on the GPU against its numpy restatement (oracle/guide_ref.py):
+MFMA inference, the fused training gradient, the Adam step, dynamic loss
+scaling, and the guided renderer's unbiasedness and record capture.
+
+bf16 tolerances: inputs, weights and activations are rounded to bfloat16
+(8 significant bits) as the kernels consume them, and the oracle emulates that
+rounding; what remains is fp32 vs float64 accumulation order, which can flip
+one bf16 rounding of an activation (2^-8 relative) now and then."""
+import numpy as np
+import pytest
+
+import guide_ref as G
+from swpathtracer_amd import api, scenes
+import swpathtracer_amd._lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+@pytest.fixture(scope="module")
+def box_scene(device):
+    sc = scenes.cornell_icosphere(subdiv=2, width=32, height=32, spp=4)
+    bvh = api.Bvh(sc.verts)
+    device.set_scene(bvh, sc.normals, sc.mat_ids)
+    device.set_materials(sc.materials)
+    inf = bvh.info
+    bmin = np.array(inf["root_min"], np.float64)
+    bext = np.maximum(np.array(inf["root_max"], np.float64) - bmin, 1e-6)
+    return sc, bmin, bext
+
+
+def _theta(seed, out_scale=10.0):
+    rng = np.random.default_rng(seed)
+    th = G.unpack(G.init_params(seed).astype(np.float64).copy())
+    th["W4"][:G.OUT] *= out_scale
+    for b in ("b1", "b2", "b3"):
+        th[b][:] = rng.normal(0, 0.1, th[b].shape)
+    th["b4"][:G.OUT] = rng.normal(0, 0.5, G.OUT)
+    return G.pack(th).astype(np.float32)
+
+
+def _queries(rng, n, bmin, bext):
+    p = bmin + rng.random((n, 3)) * bext
+    return np.concatenate([p, _unit(rng, n), _unit(rng, n)], 1).astype(np.float32)
+
+
+def _records(rng, n, bmin, bext, target=None, kappa=50.0):
+    q = _queries(rng, n, bmin, bext).astype(np.float64)
+    if target is None:
+        wi = _unit(rng, n)
+    else:
+        mu = target(q[:, 0:3])
+        wi = G.sample_vmf(mu, np.full(n, kappa), rng.random(n), rng.random(n))
+        wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    rec = np.zeros((n, L.SPT_GUIDE_RECORD_FLOATS), np.float32)
+    rec[:, 0:9] = q
+    rec[:, 9:12] = wi
+    rec[:, 12] = rng.gamma(2.0, 1.0, n) if target is None else 1.0
+    return rec
+
+
+def _encode(rec, bmin, bext):
+    r = rec.astype(np.float64)
+    return G.encode(r[:, 0:3], r[:, 3:6], r[:, 6:9], bmin, bext)
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 1000, 65536 + 7])
+def test_eval_matches_oracle(device, box_scene, n):
+    """Raw 40 outputs of the bf16-MFMA MLP vs the oracle's forward pass
+    (ragged tails: n not a multiple of the 16-row tile)."""
+    sc, bmin, bext = box_scene
+    device.guide_init(seed=5)
+    theta = _theta(11)
+    device.guide_set_params(theta)
+    rng = np.random.default_rng(n)
+    q = _queries(rng, n, bmin, bext)
+    y = device.guide_eval(q)
+    yr, _ = G.forward(theta, G.encode(q[:, 0:3].astype(np.float64), q[:, 3:6], q[:, 6:9], bmin, bext))
+    yr = yr[:, :G.OUT]
+    err = np.abs(y - yr) / (1.0 + np.abs(yr))
+    print(f"n={n} max err {err.max():.3e} mean {err.mean():.3e} |y| {np.abs(yr).mean():.3f}")
+    assert np.all(np.isfinite(y))
+    assert err.mean() < 2e-3 and err.max() < 5e-2
+    # a transposed operand or a wrong lane map breaks this by orders of magnitude
+    assert np.corrcoef(y.ravel(), yr.ravel())[0, 1] > 0.9999
+
+
+def test_gradient_matches_oracle(device, box_scene):
+    """Training gradient (KL loss, batch-mean weights, loss scale 65536)
+    vs the oracle, layer by layer."""
+    sc, bmin, bext = box_scene
+    device.guide_init(seed=5)
+    theta = _theta(12, out_scale=3.0)
+    device.guide_set_params(theta)
+    rng = np.random.default_rng(3)
+    rec = _records(rng, 4096 + 5, bmin, bext)
+    g, loss = device.guide_gradient(rec)
+    r = rec.astype(np.float64)
+    loss_r, g_r = G.batch_grad(theta, _encode(rec, bmin, bext), r[:, 9:12], r[:, 12], loss_scale=65536.0)
+    print(f"loss gpu {loss:.6f} oracle {loss_r:.6f}")
+    assert abs(loss - loss_r) <= 1e-3 * abs(loss_r)
+    gp, gr = G.unpack(g.astype(np.float64)), G.unpack(g_r)
+    for name, _ in G.SHAPES:
+        a, b = gp[name].ravel(), gr[name].ravel()
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
+        print(f"{name}: rel {rel:.3e} cos {cos:.7f}")
+        assert rel < 2e-2 and cos > 0.9995, name
+    assert np.all(g[G._pad_mask()] == 0.0)
+
+
+def test_adam_step(device, box_scene):
+    """First Adam step: m = 0.1 g, v = 0.01 g^2, bias corrections 0.1, 0.01
+    give theta -= lr g / (|g| + 1e-8)."""
+    sc, bmin, bext = box_scene
+    lr = 5e-3
+    device.guide_init(seed=5, lr=lr)
+    theta = _theta(13, out_scale=3.0)
+    device.guide_set_params(theta)
+    rec = _records(np.random.default_rng(4), 2048, bmin, bext)
+    g, _ = device.guide_gradient(rec)
+    device.guide_train(rec)
+    th1 = device.guide_get_params()
+    expect = theta.astype(np.float64) - lr * g / (np.abs(g) + 1e-8)
+    big = np.abs(g) > 1e-5  # summation order may move tiny gradients
+    np.testing.assert_allclose(th1[big], expect[big], rtol=0, atol=lr * 1e-3)
+    assert np.all(np.abs(th1 - theta) <= lr * 1.0001)
+    assert np.all(th1[G._pad_mask()] == 0.0)
+    st = device.guide_stats()
+    assert st["steps"] == 1 and st["skipped_steps"] == 0
+
+
+def test_loss_scale_backoff(device, box_scene):
+    """A loss scale that overflows the gradient skips the update and halves
+    the scale until the gradient is finite again."""
+    sc, bmin, bext = box_scene
+    device.guide_init(seed=5, loss_scale=3.0e38)
+    theta = _theta(14, out_scale=3.0)
+    device.guide_set_params(theta)
+    rec = _records(np.random.default_rng(5), 64, bmin, bext)
+    scale = 3.0e38
+    for _ in range(140):
+        device.guide_train(rec)
+        st = device.guide_stats()
+        if st["steps"]:
+            break
+        assert np.array_equal(device.guide_get_params(), theta)  # skipped: unchanged
+        scale *= 0.5
+        assert st["loss_scale"] == np.float32(scale)
+    st = device.guide_stats()
+    assert st["steps"] == 1 and st["skipped_steps"] >= 1
+    assert not np.array_equal(device.guide_get_params(), theta)
+
+
+def test_training_fits_a_directional_target(device, box_scene):
+    """Records whose directions follow a position-dependent vMF (kappa 50):
+    Adam drives the KL loss from ~log(4 pi) towards the target's entropy
+    (about -1.07 for kappa 50) and the learned density peaks on the target."""
+    sc, bmin, bext = box_scene
+    device.guide_init(seed=7, lr=5e-3)
+
+    def target(p):
+        x = (p - bmin) / bext
+        d = np.stack([x[:, 0] - 0.5, np.ones(len(p)), x[:, 2] - 0.5], 1)
+        return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+    rng = np.random.default_rng(8)
+    rec = _records(rng, 16384, bmin, bext, target=target)
+    losses = [device.guide_train(rec) for _ in range(300)]
+    print("loss", losses[0], losses[99], losses[-1])
+    assert losses[0] > 2.0 and losses[-1] < 0.0
+    q = _queries(np.random.default_rng(9), 2048, bmin, bext)
+    y = device.guide_eval(q).astype(np.float64)
+    mu = target(q[:, 0:3].astype(np.float64))
+    pdf_at = G.mixture_pdf(y, mu)
+    assert np.median(pdf_at) > 4.0  # uniform is 1 / (4 pi) = 0.08; vMF(50) peak is 8
+
+
+def _furnace(device, sc):
+    bvh = api.Bvh(sc.verts)
+    device.set_scene(bvh, sc.normals, sc.mat_ids)
+    device.set_materials(sc.materials)
+    return bvh
+
+
+@pytest.mark.parametrize("sharp", [False, True])
+def test_guided_furnace_is_unbiased(device, sharp):
+    """Lambertian sphere of albedo rho in the furnace: one bounce to the
+    emitter, so every path's expectation is 0.5 rho. Guided sampling mixes
+    0.5 BSDF + 0.5 guide; weights are bounded by 2 rho, so the mean is within
+    a few standard errors. 'sharp' sets kappa ~ 150 lobes that point partly
+    below the surface: a sampling routine that disagreed with the density
+    would bias the mean."""
+    rho = np.array([0.999, 0.5, 0.25])
+    sc = scenes.furnace(albedo=tuple(rho), width=64, height=64, spp=16)
+    _furnace(device, sc)
+    device.guide_init(seed=3)
+    if sharp:
+        th = G.unpack(G.init_params(3).astype(np.float64).copy())
+        th["W4"][:] = 0.0
+        rng = np.random.default_rng(2)
+        b4 = th["b4"]
+        for k in range(G.LOBES):
+            b4[5 * k:5 * k + 3] = _unit(rng, 1)[0]
+            b4[5 * k + 3] = np.log(150.0)
+            b4[5 * k + 4] = rng.normal()
+        device.guide_set_params(G.pack(th).astype(np.float32))
+    img, paths = device.render_image(sc.camera, sc.integrator, per_path=True, extra_flags=L.SPT_RENDER_GUIDED)
+    _, plain = device.render_image(sc.camera, sc.integrator, per_path=True)
+    obj = np.abs(plain[:, 1] - 0.5) > 0.1  # paths that met the sphere (plain paths are exactly 0.5 rho there)
+    assert obj.mean() > 0.3
+    x = paths[obj].astype(np.float64)
+    mean, se = x.mean(0), x.std(0) / np.sqrt(len(x))
+    print("mean", mean, "expect", 0.5 * rho, "se", se)
+    assert np.all(np.abs(mean - 0.5 * rho) < 5 * se + 1e-6)
+    assert np.all(x >= 0) and np.all(x <= 2 * 0.5 * rho + 1e-4)
+    assert x.std(0)[0] > 0  # the guide did sample
+
+
+def test_guided_training_render_collects_records(device):
+    sc = scenes.slit_box(rings=32, segments=64, width=48, height=36, spp=4)
+    bvh = api.Bvh(sc.verts)
+    device.set_scene(bvh, sc.normals, sc.mat_ids)
+    device.set_materials(sc.materials)
+    device.guide_init(seed=1, batch=65536)
+    img, _ = device.render_image(sc.camera, sc.integrator, extra_flags=L.SPT_RENDER_GUIDED | L.SPT_RENDER_GUIDE_TRAIN)
+    st = device.guide_stats()
+    assert st["records"] > 1000
+    rec = device.guide_records()
+    assert rec.shape[0] == min(st["records"], 65536)
+    assert np.all(np.isfinite(rec[:, :13]))
+    assert np.all(rec[:, 12] >= 0)
+    np.testing.assert_allclose(np.linalg.norm(rec[:, 3:6], axis=1), 1, atol=1e-4)
+    np.testing.assert_allclose(np.linalg.norm(rec[:, 9:12], axis=1), 1, atol=1e-4)
+    loss = device.guide_train_collected()
+    assert np.isfinite(loss)
+    st = device.guide_stats()
+    assert st["steps"] + st["skipped_steps"] == 1 and st["records"] == 0
+    assert np.all(np.isfinite(img))
