@@ -97,6 +97,117 @@ def closest_hit_bench(dev, mesh, n_rays, reps=5):
             "fetch_gbs": byts / (ms * 1e-3) / 1e9}
 
 
+def _rmse(img, ref):
+    return float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
+
+
+def guide_experiment(args):
+    """Time to equal RMSE on the slit-lit scene: the plain tracer renders
+    1024 spp (timed); the guided tracer renders progressive passes, training
+    the guide online after each pass, until its running image reaches the
+    plain image's RMSE against an FP64-film reference of oracle_spp samples
+    (independent seed). Only render and training time is counted; RMSE
+    evaluation between passes is not."""
+    from swpathtracer_amd import api, scenes
+    import swpathtracer_amd._lib as L
+
+    W, H = args.guide_res
+    t0 = time.perf_counter()
+    sc = scenes.slit_box(rings=args.rings, segments=args.segments, width=W, height=H, spp=1024)
+    dev = api.Device(0)
+    bvh = api.Bvh(sc.verts, n_threads=os.cpu_count() or 8)
+    dev.set_scene(bvh, sc.normals, sc.mat_ids)
+    dev.set_materials(sc.materials)
+    log(f"slit scene {sc.n_tris} triangles, {W}x{H} ({time.perf_counter() - t0:.1f} s)")
+    cam, integ = sc.camera, sc.integrator
+    npix = W * H
+
+    # reference image: FP64 film, independent seed
+    osp = args.oracle_spp
+    film64 = dev.alloc(npix * 3 * 8)
+    film64.zero()
+    t0 = time.perf_counter()
+    chunk = 256
+    for b in range(0, osp, chunk):
+        p = integ.params()
+        p.spp, p.seed = osp, 0x5EED0F0A
+        p.spp_begin, p.spp_count = b, min(chunk, osp - b)
+        p.flags = L.SPT_RENDER_FILM64
+        dev.render(cam, p, None, film64.ptr, None)
+    dev.sync()
+    ref = film64.download(np.float64, (H, W, 3)) / osp
+    log(f"reference {osp} spp: {time.perf_counter() - t0:.1f} s, mean {ref.reshape(-1, 3).mean(0)}")
+
+    film = dev.alloc(npix * 3 * 4)
+
+    def plain(spp):
+        film.zero()
+        p = integ.params()
+        p.spp = spp
+        p.flags = L.SPT_RENDER_FILM32
+        dev.sync()
+        t = time.perf_counter()
+        dev.render(cam, p, film.ptr, None, None)
+        dev.sync()
+        return time.perf_counter() - t, film.download(np.float32, (H, W, 3)) / spp
+
+    plain(16)  # warm-up
+    t_plain, img_plain = plain(1024)
+    r_plain = _rmse(img_plain, ref)
+    log(f"plain 1024 spp: {t_plain * 1e3:.1f} ms, RMSE {r_plain:.5f}")
+
+    # guided: progressive passes with online training
+    dev.guide_init(seed=1, lr=args.guide_lr, batch=262144)
+    film.zero()
+    total_spp = 4096
+    done, t_guided, t_train, hit = 0, 0.0, 0.0, None
+    curve = []
+    # each pass is a complete jittered estimate of c spp (c square) with its own seed
+    chunk_sched = [1] * 16 + [4] * 8
+    k = 0
+    while done < total_spp:
+        c = chunk_sched[k] if k < len(chunk_sched) else 16
+        k += 1
+        p = integ.params()
+        p.spp, p.seed = c, 0x600D0000 + k
+        p.flags = L.SPT_RENDER_FILM32 | L.SPT_RENDER_GUIDED | L.SPT_RENDER_GUIDE_TRAIN
+        dev.sync()
+        t = time.perf_counter()
+        dev.render(cam, p, film.ptr, None, None)
+        t1 = time.perf_counter()
+        dev.guide_train_collected()
+        dev.sync()
+        t2 = time.perf_counter()
+        t_guided += t2 - t
+        t_train += t2 - t1
+        done += c
+        img = film.download(np.float32, (H, W, 3)) / done
+        r = _rmse(img, ref)
+        curve.append((done, round(t_guided * 1e3, 2), round(r, 6)))
+        if hit is None and r <= r_plain:
+            hit = (done, t_guided, r)
+            if not args.guide_full:
+                break
+        if t_guided > 3 * t_plain and hit is None and not args.guide_full:
+            break
+    st = dev.guide_stats()
+    flops = st["trained_records"] * 76.8e3
+    mfma = flops / max(st["train_kernel_ms"] * 1e-3, 1e-12) / 2.5e15
+    res = {"scene": f"slit box, {sc.n_tris} triangles, {W}x{H}", "reference_spp": osp,
+           "plain_ms": round(t_plain * 1e3, 2), "plain_rmse": round(r_plain, 6),
+           "plain_mpaths_s": round(npix * 1024 / t_plain / 1e6, 3),
+           "guided_spp_to_equal_rmse": hit[0] if hit else None,
+           "guided_ms_to_equal_rmse": round(hit[1] * 1e3, 2) if hit else None,
+           "time_ratio": round(hit[1] / t_plain, 4) if hit else None,
+           "guided_train_ms": round(t_train * 1e3, 2), "train_steps": st["steps"],
+           "train_kernel_ms": round(st["train_kernel_ms"], 3), "trained_records": st["trained_records"],
+           "train_mfma_util": round(mfma, 6), "final_loss": st["last_loss"],
+           "curve": curve[:: max(1, len(curve) // 24)]}
+    log(f"guide experiment: {res}")
+    dev.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +218,12 @@ def main():
     ap.add_argument("--no-closest-hit", action="store_true")
     ap.add_argument("--rings", type=int, default=512)
     ap.add_argument("--segments", type=int, default=1024)
+    ap.add_argument("--guide", action="store_true", help="also run the guided vs plain equal-RMSE experiment")
+    ap.add_argument("--guide-only", action="store_true")
+    ap.add_argument("--guide-full", action="store_true", help="keep rendering after equal RMSE (full curve)")
+    ap.add_argument("--guide-res", type=int, nargs=2, default=(800, 600))
+    ap.add_argument("--oracle-spp", type=int, default=16384)
+    ap.add_argument("--guide-lr", type=float, default=5e-3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -129,6 +246,10 @@ def main():
             dev._check(dev.lib.spt_memcpy_d2d(dev.h, b.ptr, a.ptr, nbytes))
         dev.sync()
         print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
+        return
+
+    if args.guide_only:
+        print(json.dumps({"guide_experiment": guide_experiment(args)}), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -195,6 +316,8 @@ def main():
         ch = closest_hit_bench(dev, mesh, 16_777_216)
         log(f"closest hit: {ch}")
         extra["closest_hit_incoherent"] = {k: round(v, 4) if isinstance(v, float) else v for k, v in ch.items()}
+    if args.guide:
+        extra["guide_experiment"] = guide_experiment(args)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sc)

@@ -304,6 +304,7 @@ typedef struct spt_render_stats {
     uint64_t extend_launches;
     uint64_t node_fetches;
     uint64_t tri_fetches;
+    double guide_ms;       /* with SPT_RENDER_TIME_KERNELS and GUIDED: guide inference time */
 } spt_render_stats;
 spt_status spt_render_get_stats(spt_ctx* ctx, spt_render_stats* out);
 
@@ -311,6 +312,81 @@ spt_status spt_render_get_stats(spt_ctx* ctx, spt_render_stats* out);
  * slot in [0, 8); spt_event_elapsed() waits for slot b and returns ms a -> b. */
 spt_status spt_event_record(spt_ctx* ctx, int slot);
 spt_status spt_event_elapsed(spt_ctx* ctx, int a, int b, float* ms);
+
+/* ------------------------------------------------------------------------- */
+/* Launch tuning and the autotuner (no reference counterpart)                */
+/* ------------------------------------------------------------------------- */
+
+/* Queue and tile sizes of the wavefront pipeline. A zero field means the
+ * default. Results never depend on these, only speed. */
+typedef struct spt_tuning {
+    uint32_t path_capacity;        /* paths in flight (queue size; default 4,194,304) */
+    uint32_t extend_blocks_per_cu; /* persistent traversal grid (default: occupancy limit) */
+    uint32_t shade_blocks_per_cu;  /* shading grid (default 8) */
+    uint32_t use_graph;            /* 1: bounce loop captured once in a HIP graph and replayed;
+                                      2: plain launches (default) */
+    uint32_t tile_pixels;          /* multi-scene calls: pixels per generation tile (default 4096) */
+    uint32_t tile_samples;         /* multi-scene calls: samples per tile (default 16) */
+    uint32_t check_every;          /* bounces between reads of the live count (default 4) */
+    uint32_t pad;
+} spt_tuning;
+spt_status spt_set_tuning(spt_ctx* ctx, const spt_tuning* t);
+spt_status spt_get_tuning(spt_ctx* ctx, spt_tuning* t);
+/* Times candidate tunings on a sample of the given render (about sample_paths
+ * paths per trial; outputs discarded), keeps the fastest in the context and
+ * returns it. Coordinate descent over capacity, grids, graph and check
+ * interval. */
+spt_status spt_autotune(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* params, uint64_t sample_paths,
+                        spt_tuning* out);
+
+/* ------------------------------------------------------------------------- */
+/* Batched rendering of many scenes (no reference counterpart)               */
+/* ------------------------------------------------------------------------- */
+
+/* Scenes of any size are concatenated in one set of device arrays (tree
+ * indices and material ids rebased per scene) and rendered by one wavefront:
+ * every bounce is one launch of each stage over the live paths of all scenes,
+ * the traversal grid is persistent, and paths are generated in tiles taken
+ * round-robin from the scenes. Per path the result equals a single-scene
+ * spt_render of the same scene with the same parameters. GUIDED and replay
+ * streams are per-scene options the batch does not take; max_depth must not be 0. */
+typedef struct spt_batch_scene {
+    const spt_bvh* bvh;
+    const float* verts;      /* n_prims x 9 (as spt_set_scene) */
+    const float* normals;    /* or NULL */
+    const uint32_t* mat_ids; /* or NULL */
+    const spt_material* mats;
+    uint32_t n_mats;
+    uint32_t pad;
+    spt_camera camera;
+    spt_render_params params; /* FILM32 / FILM64 / PER_PATH flags select the outputs below */
+    float* d_rgb32;
+    double* d_rgb64;
+    float* d_path_rgb;
+} spt_batch_scene;
+
+typedef struct spt_batch spt_batch;
+
+typedef struct spt_batch_stats {
+    uint64_t scenes;
+    uint64_t paths;
+    uint64_t rays;
+    uint64_t bounces;
+    double ms;                /* host wall time of the render call */
+    uint64_t tiles;
+    uint64_t triangles;
+    uint64_t device_bytes;    /* scene arrays of the batch */
+} spt_batch_stats;
+
+spt_status spt_batch_create(spt_ctx* ctx, const spt_batch_scene* scenes, uint32_t n, spt_batch** out);
+/* Renders every scene of the batch (tuning: the context's). */
+spt_status spt_batch_render(spt_ctx* ctx, spt_batch* batch);
+/* scene_ms (n scenes, may be NULL): time from the start of the last
+ * spt_batch_render until each scene's last path finished (device clock). */
+spt_status spt_batch_get_stats(spt_ctx* ctx, const spt_batch* batch, spt_batch_stats* out, double* scene_ms);
+/* As spt_autotune, on a sample of every scene's samples (1/sample_div of each). */
+spt_status spt_batch_autotune(spt_ctx* ctx, spt_batch* batch, uint32_t sample_div, spt_tuning* out);
+void spt_batch_destroy(spt_batch* batch);
 
 /* ------------------------------------------------------------------------- */
 /* Learned path guide (no reference counterpart)                             */
@@ -345,6 +421,12 @@ typedef struct spt_guide_stats {
     float last_loss;
     float loss_scale;
     double train_ms;       /* host wall time spent in training steps */
+    double train_kernel_ms;/* device time of the fused training kernel (HIP events) */
+    uint64_t trained_records; /* records consumed by successful steps */
+    uint64_t candidates;   /* guided vertices of the last training render */
+    uint32_t record_stride;/* 1 in record_stride candidates is recorded (set from the
+                              previous render so a pass fills about one batch) */
+    uint32_t pad;
 } spt_guide_stats;
 
 /* (Re)initialises the guide (cfg NULL: defaults). */

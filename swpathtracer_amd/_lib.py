@@ -90,7 +90,8 @@ class BvhInfo(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("bounces", C.c_uint64),
                 ("stream_overflows", C.c_uint64), ("ms", C.c_double), ("extend_ms", C.c_double),
-                ("extend_launches", C.c_uint64), ("node_fetches", C.c_uint64), ("tri_fetches", C.c_uint64)]
+                ("extend_launches", C.c_uint64), ("node_fetches", C.c_uint64), ("tri_fetches", C.c_uint64),
+                ("guide_ms", C.c_double)]
 
 
 class GuideConfig(C.Structure):
@@ -100,7 +101,28 @@ class GuideConfig(C.Structure):
 
 class GuideStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("skipped_steps", C.c_uint64), ("records", C.c_uint64),
-                ("last_loss", C.c_float), ("loss_scale", C.c_float), ("train_ms", C.c_double)]
+                ("last_loss", C.c_float), ("loss_scale", C.c_float), ("train_ms", C.c_double),
+                ("train_kernel_ms", C.c_double), ("trained_records", C.c_uint64), ("candidates", C.c_uint64),
+                ("record_stride", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("path_capacity", C.c_uint32), ("extend_blocks_per_cu", C.c_uint32),
+                ("shade_blocks_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("tile_pixels", C.c_uint32),
+                ("tile_samples", C.c_uint32), ("check_every", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class BatchScene(C.Structure):
+    _fields_ = [("bvh", C.c_void_p), ("verts", C.c_void_p), ("normals", C.c_void_p), ("mat_ids", C.c_void_p),
+                ("mats", C.POINTER(Material)), ("n_mats", C.c_uint32), ("pad", C.c_uint32),
+                ("camera", Camera), ("params", RenderParams), ("d_rgb32", C.c_void_p), ("d_rgb64", C.c_void_p),
+                ("d_path_rgb", C.c_void_p)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("scenes", C.c_uint64), ("paths", C.c_uint64), ("rays", C.c_uint64), ("bounces", C.c_uint64),
+                ("ms", C.c_double), ("tiles", C.c_uint64), ("triangles", C.c_uint64),
+                ("device_bytes", C.c_uint64)]
 
 
 assert C.sizeof(Layer) == 64 and C.sizeof(Material) == 272
@@ -144,6 +166,15 @@ PROTOTYPES = {
     "spt_guide_train_collected": (C.c_int32, [C.c_void_p, C.POINTER(C.c_float)]),
     "spt_guide_get_stats": (C.c_int32, [C.c_void_p, C.POINTER(GuideStats)]),
     "spt_guide_records": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
+    "spt_set_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
+    "spt_get_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
+    "spt_autotune": (C.c_int32, [C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams), C.c_uint64,
+                                 C.POINTER(Tuning)]),
+    "spt_batch_create": (C.c_int32, [C.c_void_p, C.POINTER(BatchScene), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "spt_batch_render": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "spt_batch_get_stats": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(BatchStats), C.c_void_p]),
+    "spt_batch_autotune": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(Tuning)]),
+    "spt_batch_destroy": (None, [C.c_void_p]),
 }
 
 _lib = None

@@ -338,6 +338,24 @@ class Device:
     def render(self, cam: L.Camera, params: L.RenderParams, d_rgb32=None, d_rgb64=None, d_path=None):
         self._check(self.lib.spt_render(self.h, C.byref(cam), C.byref(params), d_rgb32, d_rgb64, d_path))
 
+    # tuning ------------------------------------------------------------
+    def set_tuning(self, **kw):
+        t = L.Tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._check(self.lib.spt_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = L.Tuning()
+        self._check(self.lib.spt_get_tuning(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_ if f != "pad"}
+
+    def autotune(self, cam: L.Camera, integ: "Integrator", sample_paths: int = 8 << 20) -> dict:
+        p = integ.params()
+        t = L.Tuning()
+        self._check(self.lib.spt_autotune(self.h, C.byref(cam), C.byref(p), sample_paths, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_ if f != "pad"}
+
     # learned guide -----------------------------------------------------
     def guide_init(self, seed: int = 1, lr: float = 5e-3, loss_scale: float = 65536.0, bsdf_fraction: float = 0.5,
                    batch: int = 262144):
@@ -441,3 +459,88 @@ class Device:
         img = img / (np.float64(cnt) if film64 else np.float32(cnt))
         paths = dpath.download(np.float32, (vw * vh * cnt, 3)) if dpath else None
         return img, paths
+
+
+class Batch:
+    """Many scenes rendered by one wavefront (spt_batch_*). `scenes` are
+    SceneData-like objects (verts, normals, mat_ids, materials, camera,
+    integrator). Each scene gets an FP32 film, and per-path radiance when
+    per_path is set."""
+
+    def __init__(self, dev: Device, scenes, per_path: bool = False, film64: bool = False, n_threads: int = 8):
+        self.dev, self.lib = dev, dev.lib
+        self.scenes = list(scenes)
+        self.bvhs, self.keep, self.films, self.films64, self.paths = [], [], [], [], []
+        arr = (L.BatchScene * len(self.scenes))()
+        for i, sc in enumerate(self.scenes):
+            bvh = Bvh(sc.verts, n_threads=n_threads)
+            self.bvhs.append(bvh)
+            nrm = None if sc.normals is None else np.ascontiguousarray(sc.normals, np.float32).reshape(-1, 9)
+            mid = np.ascontiguousarray(sc.mat_ids, np.uint32)
+            mats = (L.Material * len(sc.materials))(*sc.materials)
+            self.keep += [nrm, mid, mats]
+            cam, p = sc.camera, sc.integrator.params()
+            npix = cam.width * cam.height
+            film = dev.alloc(npix * 12)
+            film.zero()
+            self.films.append(film)
+            p.flags = L.SPT_RENDER_FILM32
+            f64 = None
+            if film64:
+                f64 = dev.alloc(npix * 24)
+                f64.zero()
+                p.flags |= L.SPT_RENDER_FILM64
+            self.films64.append(f64)
+            pp = None
+            if per_path:
+                pp = dev.alloc(npix * sc.integrator.effective_spp() * 12)
+                p.flags |= L.SPT_RENDER_PER_PATH
+            self.paths.append(pp)
+            e = arr[i]
+            e.bvh, e.verts, e.normals, e.mat_ids = bvh.h, bvh.verts.ctypes.data, _ptr(nrm), mid.ctypes.data
+            e.mats, e.n_mats = mats, len(sc.materials)
+            e.camera, e.params = cam, p
+            e.d_rgb32 = film.ptr
+            e.d_rgb64 = f64.ptr if f64 is not None else None
+            e.d_path_rgb = pp.ptr if pp is not None else None
+        h = C.c_void_p()
+        dev._check(self.lib.spt_batch_create(dev.h, arr, len(self.scenes), C.byref(h)))
+        self.h = h
+
+    def render(self):
+        self.dev._check(self.lib.spt_batch_render(self.dev.h, self.h))
+
+    def autotune(self, sample_div: int = 16) -> dict:
+        t = L.Tuning()
+        self.dev._check(self.lib.spt_batch_autotune(self.dev.h, self.h, sample_div, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_ if f != "pad"}
+
+    def stats(self):
+        st = L.BatchStats()
+        ms = np.zeros(len(self.scenes), np.float64)
+        self.dev._check(self.lib.spt_batch_get_stats(self.dev.h, self.h, C.byref(st), ms.ctypes.data))
+        return {f: getattr(st, f) for f, _ in L.BatchStats._fields_}, ms
+
+    def clear(self):
+        for f in self.films + [f for f in self.films64 if f is not None]:
+            f.zero()
+
+    def image(self, i: int) -> np.ndarray:
+        cam = self.scenes[i].camera
+        return self.films[i].download(np.float32, (cam.height, cam.width, 3))
+
+    def path_radiance(self, i: int) -> np.ndarray:
+        sc = self.scenes[i]
+        n = sc.camera.width * sc.camera.height * sc.integrator.effective_spp()
+        return self.paths[i].download(np.float32, (n, 3))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.spt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -106,6 +106,39 @@ uint32_t tree_depth(const spt_node64* nodes, uint32_t n) {
 }
 
 }  // namespace
+
+// Triangles in leaf order with their material ids, and the 12-float shading
+// records (normals, material, smooth flag) the kernels read per hit.
+void scene_arrays(const spt_bvh* bvh, const float* verts, const float* normals, const uint32_t* mat_ids,
+                  std::vector<spt_tri48>& tris, std::vector<float>& sh) {
+    tris = bvh_tris(bvh);
+    const std::vector<uint32_t>& order = bvh_leaf_prim(bvh);
+    sh.assign((size_t)tris.size() * 12, 0.0f);
+    for (size_t i = 0; i < tris.size(); ++i) {
+        const uint32_t p = order[i];
+        const uint32_t m = mat_ids ? mat_ids[p] : 0u;
+        tris[i].pad[0] = m;
+        float* o = &sh[i * 12];
+        uint32_t smooth = 0u;
+        if (normals)
+            for (int j = 0; j < 9; ++j) smooth |= (normals[9ull * p + j] != 0.0f) ? 1u : 0u;
+        if (smooth) {
+            std::memcpy(o, normals + 9ull * p, 9 * sizeof(float));
+        } else {
+            const float* v = verts + 9ull * p;
+            V3 a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
+            V3 n = normalize(cross(b - a, c - a));  // triangle.cpp:5-6
+            o[0] = n.x;
+            o[1] = n.y;
+            o[2] = n.z;
+        }
+        std::memcpy(&o[9], &m, 4);
+        std::memcpy(&o[10], &smooth, 4);
+    }
+}
+
+uint32_t tree_stack_cap(const spt_node64* nodes, uint32_t n) { return tree_depth(nodes, n) + 1; }
+
 }  // namespace spt
 
 using namespace spt;
@@ -127,6 +160,9 @@ spt_status spt_create(int device, spt_ctx** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        ctx->wall_clock_khz = khz;
     if (hipMalloc(&ctx->d_flags, 64) != hipSuccess || hipMemset(ctx->d_flags, 0, 64) != hipSuccess) {
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -154,6 +190,8 @@ void spt_destroy(spt_ctx* ctx) {
     if (ctx->guide.d_theta) (void)hipFree(ctx->guide.d_theta);
     if (ctx->guide.d_records) (void)hipFree(ctx->guide.d_records);
     if (ctx->guide.d_gout) (void)hipFree(ctx->guide.d_gout);
+    for (hipEvent_t e : ctx->guide.ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->d_guidable) (void)hipFree(ctx->d_guidable);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -281,30 +319,9 @@ spt_status spt_set_scene(spt_ctx* ctx, const spt_bvh* bvh, const float* verts, c
                          const uint32_t* mat_ids) {
     if (!ctx || !bvh || !verts) return SPT_ERR_INVALID;
     const std::vector<spt_node64>& nodes = bvh_nodes(bvh);
-    std::vector<spt_tri48> tris = bvh_tris(bvh);
-    const std::vector<uint32_t>& order = bvh_leaf_prim(bvh);
-    std::vector<float> sh((size_t)tris.size() * 12, 0.0f);
-    for (size_t i = 0; i < tris.size(); ++i) {
-        const uint32_t p = order[i];
-        const uint32_t m = mat_ids ? mat_ids[p] : 0u;
-        tris[i].pad[0] = m;
-        float* o = &sh[i * 12];
-        uint32_t smooth = 0u;
-        if (normals)
-            for (int j = 0; j < 9; ++j) smooth |= (normals[9ull * p + j] != 0.0f) ? 1u : 0u;
-        if (smooth) {
-            std::memcpy(o, normals + 9ull * p, 9 * sizeof(float));
-        } else {
-            const float* v = verts + 9ull * p;
-            V3 a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
-            V3 n = normalize(cross(b - a, c - a));  // triangle.cpp:5-6
-            o[0] = n.x;
-            o[1] = n.y;
-            o[2] = n.z;
-        }
-        std::memcpy(&o[9], &m, 4);
-        std::memcpy(&o[10], &smooth, 4);
-    }
+    std::vector<spt_tri48> tris;
+    std::vector<float> sh;
+    scene_arrays(bvh, verts, normals, mat_ids, tris, sh);
     return spt_upload_scene(ctx, nodes.data(), (uint32_t)nodes.size(), tris.data(), (uint32_t)tris.size(),
                             bvh_root_box(bvh), sh.data());
 }

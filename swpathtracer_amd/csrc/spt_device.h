@@ -190,6 +190,18 @@ __device__ inline void trace_ray(const DevScene& sc, V3 o, V3 d, Stack& st, Hit&
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kDone = 0xFFFFFFFFu;
 
+// Where traversal starts: the root box, and either a root leaf of `leaf`
+// triangles at tri0 or the inner node node0. One per scene; rays of several
+// scenes share the node and triangle arrays (indices rebased per scene).
+struct Root {
+    V3 mn, mx;
+    uint32_t leaf, node0, tri0;
+};
+
+__device__ inline Root scene_root(const DevScene& sc) {
+    return Root{V3{sc.rmin[0], sc.rmin[1], sc.rmin[2]}, V3{sc.rmax[0], sc.rmax[1], sc.rmax[2]}, sc.root_leaf, 0u, 0u};
+}
+
 template <bool FETCH>
 struct PLane {
     V3 o, d, inv;
@@ -200,11 +212,12 @@ struct PLane {
     uint32_t nodes, tris;  // FETCH counters
 };
 
-// GetRay(i, o, d) loads ray i; PutHit(i, h, nodes, tris) stores its result.
+// GetRay(i, o, d, root) loads ray i (and may set the root of its scene; it
+// starts as the single scene's root); PutHit(i, h, nodes, tris) stores its result.
 template <bool FETCH, typename GetRay, typename PutHit>
 __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t* ray_counter, Stack& st,
                                         bool& overflow, GetRay get_ray, PutHit put_hit) {
-    const V3 rmn{sc.rmin[0], sc.rmin[1], sc.rmin[2]}, rmx{sc.rmax[0], sc.rmax[1], sc.rmax[2]};
+    Root R = scene_root(sc);
     const uint64_t lt = __lanemask_lt();
     PLane<FETCH> L;
     L.ray = kDone;
@@ -224,7 +237,7 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                 const uint32_t i = base + (uint32_t)__popcll(idle & lt);
                 if (i < n) {
                     L.ray = i;
-                    get_ray(i, L.o, L.d);
+                    get_ray(i, L.o, L.d, R);
                     L.inv = V3{1.0f / L.d.x, 1.0f / L.d.y, 1.0f / L.d.z};
                     L.h.t = __builtin_inff();
                     L.h.idx = kNoHit;
@@ -234,10 +247,10 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                     L.sp = 0;
                     if (FETCH) L.nodes = L.tris = 0;
                     float tn, tf;
-                    if (!slab(L.o, L.inv, rmn, rmx, tn, tf))
+                    if (!slab(L.o, L.inv, R.mn, R.mx, tn, tf))
                         L.cur = kDone;
                     else
-                        L.cur = sc.root_leaf ? kLeafBit : 0u;
+                        L.cur = R.leaf ? kLeafBit : R.node0;
                 }
             }
         }
@@ -281,11 +294,11 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
         while (L.ray != kDone && L.cur != kDone && (L.cur & kLeafBit)) {
             uint32_t first, cnt;
             V3 bmn, bmx;
-            if (sc.root_leaf) {
-                first = 0;
-                cnt = sc.root_leaf;
-                bmn = rmn;
-                bmx = rmx;
+            if (R.leaf) {
+                first = R.tri0;
+                cnt = R.leaf;
+                bmn = R.mn;
+                bmx = R.mx;
             } else {
                 // whole 64-B record (cache-resident), then select the side in registers
                 const uint32_t node = (L.cur & ~kLeafBit) >> 1;
@@ -359,6 +372,51 @@ __device__ inline uint32_t wave_add(uint32_t* counters, uint32_t key) {
     return slot;
 }
 
+// One scene's share of a render call: camera, sampling, termination,
+// outputs and where its tree starts in the (shared) node/triangle arrays.
+struct Job {
+    DevCamera cam;
+    uint32_t sampler, n_sub, spp, spp_begin, spp_count;
+    float w_sub;
+    int32_t max_depth, rr_depth;
+    uint64_t seed;
+    uint32_t vt, vl, vw, vh;
+    float bg[3];
+    const float* stream;
+    uint32_t stride;
+    float* film32;
+    double* film64;
+    float* path_rgb;
+    float rmin[3], rmax[3];
+    uint32_t root_leaf, node0, tri0;
+    uint64_t n_paths;  // paths of this job in the call
+};
+
+// A run of consecutive paths of one job: pixels [p0, p0 + np) of its
+// viewport (row-major), samples [s0, s0 + ns) of each, pixel-major.
+struct BatchTile {
+    uint32_t job, p0, np, s0, ns, pad[3];
+};
+
+// Device arrays and per-scene jobs of a batch (spt_batch_*).
+struct BatchData {
+    float4* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    float4* d_shading = nullptr;
+    spt_material* d_mats = nullptr;
+    uint32_t n_nodes = 0, n_tris = 0, n_mats = 0, stack_cap = 0;
+    std::vector<Job> jobs;  // host copy: sampling, outputs, roots
+    Job* d_jobs = nullptr;
+    BatchTile* d_tiles = nullptr;
+    uint64_t* d_tile_start = nullptr;
+    size_t tiles_cap = 0;
+    uint32_t* d_done = nullptr;   // finished paths per job
+    uint64_t* d_clock = nullptr;  // start, then finish time per job
+    uint64_t device_bytes = 0;
+    spt_batch_stats stats{};
+    std::vector<double> scene_ms;
+};
+
 // Scene box used by the guide's input encoding.
 struct GuideBox {
     float bmin[3], bext[3];
@@ -383,6 +441,11 @@ struct GuideState {
     double train_ms = 0.0;
     float* d_gout = nullptr;  // per-path raw outputs during a guided render
     size_t gout_cap = 0;
+    uint32_t rec_stride = 1;      // 1 in rec_stride candidate vertices is recorded
+    uint64_t candidates = 0;      // candidate vertices of the last training render
+    uint64_t trained = 0;         // records consumed by successful steps
+    double kernel_ms = 0.0;       // device time of k_guide_train
+    hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 }  // namespace spt
@@ -421,6 +484,8 @@ struct spt_ctx {
     size_t paths_bytes = 0;
     spt_render_stats stats{};
     spt::GuideState guide;
+    spt_tuning tuning{};  // zero fields: defaults
+    int wall_clock_khz = 100000;
 
     spt::DevScene scene() const {
         spt::DevScene s;
@@ -455,4 +520,13 @@ GuideBox guide_box(const spt_ctx* ctx);
 spt_status guide_eval_launch(spt_ctx* ctx, const float* d_q, uint32_t stride, uint32_t n, float* d_out);
 spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* p, float* d_rgb32,
                          double* d_rgb64, float* d_path_rgb);
+spt_tuning resolve_tuning(const spt_tuning& t);
+void scene_arrays(const spt_bvh* bvh, const float* verts, const float* normals, const uint32_t* mat_ids,
+                  std::vector<spt_tri48>& tris, std::vector<float>& sh);
+uint32_t tree_stack_cap(const spt_node64* nodes, uint32_t n);
+spt_status make_job(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* prm, float* d_rgb32, double* d_rgb64,
+                    float* d_path_rgb, Job& jb, uint64_t& total);
+// A multi-scene render (spt_batch_*): every job of the batch, or with
+// sample_div > 1 the first 1/sample_div of each job's samples.
+spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div);
 }  // namespace spt

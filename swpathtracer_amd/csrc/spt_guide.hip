@@ -224,8 +224,10 @@ spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, floa
     if (s != SPT_OK) return s;
     hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, gs.d_acc);
     const size_t lds = train_lds_bytes();
+    if (gs.ev[0]) (void)hipEventRecord(gs.ev[0], st);
     hipLaunchKernelGGL(k_guide_train, dim3((uint32_t)ctx->num_cus), dim3(kBlock), lds, st, gs.d_theta, gs.d_wbf, d_rec,
                        n, gs.d_grad, gs.d_acc, gs.loss_scale, guide_box(ctx));
+    if (gs.ev[1]) (void)hipEventRecord(gs.ev[1], st);
     hipLaunchKernelGGL(k_guide_check, dim3(64), dim3(kBlock), 0, st, gs.d_grad, reinterpret_cast<uint32_t*>(gs.d_acc + 4));
     float acc[8];
     s = check_hip(ctx, hipMemcpyAsync(acc, gs.d_acc, sizeof(acc), hipMemcpyDeviceToHost, st), "guide acc read");
@@ -239,7 +241,12 @@ spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, floa
         ++gs.skipped;
     } else {
         ++gs.step;
+        gs.trained += n;
         gs.last_loss = acc[1];
+    }
+    if (gs.ev[0] && gs.ev[1]) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, gs.ev[0], gs.ev[1]) == hipSuccess) gs.kernel_ms += ms;
     }
     const float bc1 = 1.0f - std::pow(0.9f, (float)std::max<uint64_t>(gs.step, 1));
     const float bc2 = 1.0f - std::pow(0.99f, (float)std::max<uint64_t>(gs.step, 1));
@@ -348,7 +355,14 @@ spt_status spt_guide_init(spt_ctx* ctx, const spt_guide_config* cfg) {
     gs.batch = c.batch;
     gs.skipped = 0;
     gs.train_ms = 0.0;
+    gs.kernel_ms = 0.0;
+    gs.trained = 0;
+    gs.candidates = 0;
+    gs.rec_stride = 1;
+    gs.collected = 0;
     gs.last_loss = 0.0f;
+    for (hipEvent_t& e : gs.ev)
+        if (!e && hipEventCreate(&e) != hipSuccess) e = nullptr;
     // Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero biases; output
     // layer scaled by 0.1 so the first mixtures are broad.
     std::vector<float> th(kParams, 0.0f);
@@ -423,6 +437,11 @@ spt_status spt_guide_get_stats(spt_ctx* ctx, spt_guide_stats* out) {
     out->last_loss = gs.last_loss;
     out->loss_scale = gs.loss_scale;
     out->train_ms = gs.train_ms;
+    out->train_kernel_ms = gs.kernel_ms;
+    out->trained_records = gs.trained;
+    out->candidates = gs.candidates;
+    out->record_stride = gs.rec_stride;
+    out->pad = 0;
     return SPT_OK;
 }
 

@@ -50,19 +50,14 @@ constexpr size_t kWorkBytes = 16 + 4 + 4;
 
 struct RenderArgs {
     DevScene sc;
-    DevCamera cam;
-    uint32_t sampler, n_sub, spp, spp_begin, spp_count;
-    float w_sub;
-    int32_t max_depth, rr_depth;
-    uint64_t seed;
-    uint32_t vt, vl, vw, vh;
-    float bg[3];
+    Job job;                    // single-scene call
+    const Job* jobs;            // multi-scene call: per-job parameters (path meta.w = job)
+    const BatchTile* tiles;     // multi-scene call: generation order
+    const uint64_t* tile_start; // first path of each tile (n_tiles + 1 entries)
+    uint32_t n_tiles;
+    uint32_t* job_done;         // paths finished per job
+    uint64_t* job_clock;        // [0] start, [1 + j] finish of job j (device wall clock)
     uint32_t flags;
-    const float* stream;
-    uint32_t stride;
-    float* film32;
-    double* film64;
-    float* path_rgb;
     uint32_t keys;  // sort keys (n_mats + 1), 0 = no sort
     // learned guide
     uint32_t guided, train;
@@ -70,8 +65,9 @@ struct RenderArgs {
     float bsdf_frac;
     float* gout;              // per path slot: 40 raw network outputs
     float* records;           // training records (guide::kRecord floats each)
-    uint32_t* rec_count;
+    uint32_t* rec_count;  // [0]: records written, [1]: candidate vertices
     uint32_t rec_cap;
+    uint32_t rec_stride;  // record 1 in rec_stride candidates (hash of key and depth)
 };
 
 __device__ inline float lum(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
@@ -88,35 +84,50 @@ __device__ inline void finish_records(const RenderArgs& a, uint4 gr, float4 L) {
     }
 }
 
-__device__ inline void finish_path(const RenderArgs& a, uint32_t pix, uint64_t local, float4 L) {
-    if (a.film32) {
-        float* f = a.film32 + 3 * (size_t)pix;
+template <bool MULTI>
+__device__ inline const Job& job_of(const RenderArgs& a, uint32_t j) {
+    if (MULTI) return a.jobs[j];
+    return a.job;
+}
+
+// Film and per-path output of a finished path; in a multi-scene call also
+// counts the job's finished paths and stamps the job's finish time when its
+// last path ends. Called by every active lane of the branch together.
+template <bool MULTI>
+__device__ inline void finish_path(const RenderArgs& a, const Job& jb, uint32_t jid, uint32_t pix, uint64_t local,
+                                   float4 L) {
+    if (MULTI) {
+        const uint32_t k = wave_add(a.job_done, jid);
+        if ((uint64_t)k + 1 == jb.n_paths) a.job_clock[1 + jid] = wall_clock64();
+    }
+    if (jb.film32) {
+        float* f = jb.film32 + 3 * (size_t)pix;
         unsafeAtomicAdd(f + 0, L.x);
         unsafeAtomicAdd(f + 1, L.y);
         unsafeAtomicAdd(f + 2, L.z);
     }
-    if (a.film64) {
-        double* f = a.film64 + 3 * (size_t)pix;
+    if (jb.film64) {
+        double* f = jb.film64 + 3 * (size_t)pix;
         unsafeAtomicAdd(f + 0, (double)L.x);
         unsafeAtomicAdd(f + 1, (double)L.y);
         unsafeAtomicAdd(f + 2, (double)L.z);
     }
-    if (a.path_rgb) {
-        float* o = a.path_rgb + 3 * (size_t)local;
+    if (jb.path_rgb) {
+        float* o = jb.path_rgb + 3 * (size_t)local;
         o[0] = L.x;
         o[1] = L.y;
         o[2] = L.z;
     }
 }
 
-__device__ inline Draws make_draws(const RenderArgs& a, uint64_t key, uint32_t k) {
+__device__ inline Draws make_draws(const Job& jb, uint64_t key, uint32_t k) {
     Draws r;
     r.k = k;
     r.overflow = false;
-    r.stride = a.stride;
-    if (a.stream) {
+    r.stride = jb.stride;
+    if (jb.stream) {
         r.key = 0;
-        r.stream = a.stream + key * (uint64_t)a.stride;
+        r.stream = jb.stream + key * (uint64_t)jb.stride;
     } else {
         r.key = key;
         r.stream = nullptr;
@@ -125,47 +136,73 @@ __device__ inline Draws make_draws(const RenderArgs& a, uint64_t key, uint32_t k
 }
 
 // Camera samples for paths [base, base + n) into buffer A:
-// integrators/pathtracer.cpp:104-113.
+// integrators/pathtracer.cpp:104-113. A single-scene call numbers its paths
+// pixel-major over the viewport; a multi-scene call walks its tile list.
+// With max depth 0 (single-scene calls only) every path finishes here.
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_generate(RenderArgs a, PathBuf A, Work w, uint64_t base, uint32_t n) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s == 0) {
-        w.cnt[0] = (a.max_depth == 0) ? 0u : n;
+        w.cnt[0] = (!MULTI && a.job.max_depth == 0) ? 0u : n;
         w.cnt[1] = 0;
         w.cnt[8] = 0;
+        if (base == 0 && a.job_clock) a.job_clock[0] = wall_clock64();
     }
     if (s >= n) return;
     const uint64_t g = base + s;
-    const uint64_t pl = g / a.spp_count;
-    const uint32_t smp = a.spp_begin + (uint32_t)(g % a.spp_count);
-    const uint32_t x = a.vl + (uint32_t)(pl % a.vw), y = a.vt + (uint32_t)(pl / a.vw);
-    const uint64_t path = ((uint64_t)y * a.cam.width + x) * a.spp + smp;
-    const uint32_t pix = y * a.cam.width + x;
-    const uint64_t key = a.stream ? g : rng_key(a.seed, path);
-    Draws r = make_draws(a, key, 0);
+    uint32_t jid = 0;
+    uint64_t pl, local;
+    uint32_t smp;
+    if (MULTI) {
+        uint32_t lo = 0, hi = a.n_tiles - 1;  // last tile starting at or before g
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (a.tile_start[mid] <= g)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchTile tl = a.tiles[lo];
+        const uint64_t q = g - a.tile_start[lo];
+        jid = tl.job;
+        pl = tl.p0 + q / tl.ns;
+        smp = tl.s0 + (uint32_t)(q % tl.ns);
+        local = pl * a.jobs[jid].spp_count + (smp - a.jobs[jid].spp_begin);
+    } else {
+        pl = g / a.job.spp_count;
+        smp = a.job.spp_begin + (uint32_t)(g % a.job.spp_count);
+        local = g;
+    }
+    const Job& jb = job_of<MULTI>(a, jid);
+    const uint32_t x = jb.vl + (uint32_t)(pl % jb.vw), y = jb.vt + (uint32_t)(pl / jb.vw);
+    const uint64_t path = ((uint64_t)y * jb.cam.width + x) * jb.spp + smp;
+    const uint32_t pix = y * jb.cam.width + x;
+    const uint64_t key = jb.stream ? local : rng_key(jb.seed, path);
+    Draws r = make_draws(jb, key, 0);
     float sx, sy;
-    pixel_offset(a.sampler, a.n_sub, a.w_sub, smp, r, sx, sy);
+    pixel_offset(jb.sampler, jb.n_sub, jb.w_sub, smp, r, sx, sy);
     const float xs = x + sx, ys = y + sy;
     V3 o, d;
-    camera_ray(a.cam, xs, ys, o, d);
+    camera_ray(jb.cam, xs, ys, o, d);
     if (r.overflow) atomicAdd(&w.cnt[2], 1u);
-    if (a.max_depth == 0) {  // depth >= path_length before the first trace
-        finish_path(a, pix, g, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    if (jb.max_depth == 0) {  // depth >= path_length before the first trace
+        finish_path<MULTI>(a, jb, jid, pix, local, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         return;
     }
     A.o[s] = make_float4(o.x, o.y, o.z, 0.0f);
     A.d[s] = make_float4(d.x, d.y, d.z, 0.0f);
     A.thr[s] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     A.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    A.meta[s] = make_uint4(pix, r.k, 0u, 0u);
+    A.meta[s] = make_uint4(pix, r.k, 0u, jid);
     A.key[s] = key;
-    A.local[s] = g;
+    A.local[s] = local;
     if (a.train) A.grec[s] = make_uint4(kNoRec, kNoRec, kNoRec, 0u);
 }
 
 // Closest hit of every live path of buffer A (count *cntA). Persistent
 // traversal; hit record and sort key written at the path's slot. FETCH adds
 // node / triangle record fetch counts to the two 64-bit counters at flags + 4.
-template <bool FETCH>
+template <bool FETCH, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, PathBuf A, Work w, const uint32_t* __restrict__ cntA,
                                                    uint32_t* __restrict__ spill, uint32_t* __restrict__ flags) {
     __shared__ uint32_t lds[kLdsStack * kBlock];
@@ -187,10 +224,15 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, PathBuf A, Work
     }
     trace_persistent<FETCH>(
         a.sc, n, w.cnt + 8, st, overflow,
-        [&](uint32_t i, V3& o, V3& d) {
+        [&](uint32_t i, V3& o, V3& d, Root& root) {
             const float4 o4 = A.o[i], d4 = A.d[i];
             o = V3{o4.x, o4.y, o4.z};
             d = V3{d4.x, d4.y, d4.z};
+            if (MULTI) {
+                const Job& jb = a.jobs[A.meta[i].w];
+                root = Root{V3{jb.rmin[0], jb.rmin[1], jb.rmin[2]}, V3{jb.rmax[0], jb.rmax[1], jb.rmax[2]},
+                            jb.root_leaf, jb.node0, jb.tri0};
+            }
         },
         [&](uint32_t i, const Hit& h, uint32_t nn, uint32_t nt) {
             w.hit[i] = make_float4(h.t, __uint_as_float(h.idx), h.u, h.v);
@@ -373,6 +415,7 @@ __device__ inline float layer_density(const spt_layer& l, V3 wo, V3 wi) {
 
 // Shading of one bounce (integrators/pathtracer.cpp:21-54) for every live
 // path of A, in material order when sorted; survivors are appended to B.
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathBuf B, Work w,
                                                   const uint32_t* __restrict__ cntA, uint32_t* __restrict__ cntB) {
     const uint32_t n = *cntA;
@@ -383,16 +426,17 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         float4 rad = A.rad[i];
         const float4 thr = A.thr[i];
         const uint4 meta = A.meta[i];
+        const Job& jb = job_of<MULTI>(a, meta.w);
         const uint32_t depth = meta.z;
         const uint32_t idx = __float_as_uint(hv.y);
         if (idx == kNoHit) {  // miss: background at depth 0, else 0 (pathtracer.cpp:21-26)
             if (depth == 0) {
-                rad.x += thr.x * a.bg[0];
-                rad.y += thr.y * a.bg[1];
-                rad.z += thr.z * a.bg[2];
+                rad.x += thr.x * jb.bg[0];
+                rad.y += thr.y * jb.bg[1];
+                rad.z += thr.z * jb.bg[2];
             }
             if (a.train) finish_records(a, A.grec[i], rad);
-            finish_path(a, meta.x, A.local[i], rad);
+            finish_path<MULTI>(a, jb, meta.w, meta.x, A.local[i], rad);
             continue;
         }
         const float4 o4 = A.o[i], d4 = A.d[i];
@@ -409,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         const Frame fr = frame_from_up(nrm);          // pathtracer.cpp:28
         const V3 wo = normalize(to_local(fr, -d));    // pathtracer.cpp:30
         const uint64_t key = A.key[i];
-        Draws r = make_draws(a, key, meta.y);
+        Draws r = make_draws(jb, key, meta.y);
         V3 bsdf, refl, wi;
         float pdf;
         const bool guided = a.guided && a.guidable[mid];
@@ -446,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         const float kk = gmax(gmax(refl.x, refl.y), refl.z);  // pathtracer.cpp:41
         const float rec_w = lum(bsdf) * wi.y / pdf;           // training weight, before roulette
         bool alive = true;
-        if ((a.rr_depth >= 0) && ((int32_t)depth >= a.rr_depth)) {  // pathtracer.cpp:43-49
+        if ((jb.rr_depth >= 0) && ((int32_t)depth >= jb.rr_depth)) {  // pathtracer.cpp:43-49
             if ((r.next() < kk) && (depth < 1000u))
                 bsdf = bsdf / kk;
             else
@@ -454,16 +498,23 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         }
         if (r.overflow) atomicAdd(&w.cnt[2], 1u);
         const uint32_t nd = depth + 1;
-        if (alive && (a.max_depth >= 0) && ((int32_t)nd >= a.max_depth)) alive = false;  // pathtracer.cpp:18
+        if (alive && (jb.max_depth >= 0) && ((int32_t)nd >= jb.max_depth)) alive = false;  // pathtracer.cpp:18
         if (!alive) {
             if (a.train) finish_records(a, A.grec[i], rad);
-            finish_path(a, meta.x, A.local[i], rad);
+            finish_path<MULTI>(a, jb, meta.w, meta.x, A.local[i], rad);
             continue;
         }
         // pathtracer.cpp:52-54, forwards: throughput *= f * cos / pdf
         const V3 wgt = (bsdf * wi.y) / pdf;
         uint4 gr = a.train ? A.grec[i] : make_uint4(0u, 0u, 0u, 0u);
-        if (a.train && guided && gr.w < 3u && wi.y > 0.0f && bsdf.x + bsdf.y + bsdf.z > 0.0f) {
+        const bool cand = a.train && guided && gr.w < 3u && wi.y > 0.0f && bsdf.x + bsdf.y + bsdf.z > 0.0f;
+        if (a.train) {  // candidates counted per wave (one atomic)
+            const uint64_t cb = __ballot(cand);
+            if (cb && __lane_id() == (unsigned)(__ffsll((unsigned long long)cb) - 1))
+                atomicAdd(a.rec_count + 1, (uint32_t)__popcll(cb));
+        }
+        if (cand && (a.rec_stride <= 1u ||
+                     mix64(key + (uint64_t)(depth + 1) * 0x9E3779B97F4A7C15ull) % a.rec_stride == 0u)) {
             const uint32_t rs = atomicAdd(a.rec_count, 1u);
             if (rs < a.rec_cap) {
                 float* rec = a.records + (size_t)rs * guide::kRecord;
@@ -483,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         B.d[slot] = make_float4(wwi.x, wwi.y, wwi.z, 0.0f);
         B.thr[slot] = make_float4(thr.x * wgt.x, thr.y * wgt.y, thr.z * wgt.z, 0.0f);
         B.rad[slot] = rad;
-        B.meta[slot] = make_uint4(meta.x, r.k, nd, 0u);
+        B.meta[slot] = make_uint4(meta.x, r.k, nd, meta.w);
         B.key[slot] = key;
         B.local[slot] = A.local[i];
         if (a.train) B.grec[slot] = gr;
@@ -507,78 +558,87 @@ size_t guide_paths_lds_bytes() {
 
 }  // namespace
 
-spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* prm, float* d_rgb32,
-                         double* d_rgb64, float* d_path_rgb) {
-    auto t0 = std::chrono::steady_clock::now();
-    RenderArgs a{};
-    a.sc = ctx->scene();
-    a.cam = make_camera(*cam);
-    a.sampler = prm->pixel_sampler;
-    if (a.sampler > SPT_PIXEL_UNIFORM) return set_error(ctx, SPT_ERR_INVALID, "unknown pixel sampler");
+// Defaults for the zero fields of a tuning.
+spt_tuning resolve_tuning(const spt_tuning& t) {
+    spt_tuning r = t;
+    if (r.path_capacity == 0) r.path_capacity = 1u << 22;
+    if (r.shade_blocks_per_cu == 0) r.shade_blocks_per_cu = 8;
+    if (r.use_graph == 0) r.use_graph = 2;
+    if (r.tile_pixels == 0) r.tile_pixels = 4096;
+    if (r.tile_samples == 0) r.tile_samples = 16;
+    if (r.check_every == 0) r.check_every = 4;
+    return r;
+}
+
+// Validates one render's parameters and fills its job (everything but the
+// tree root). total = paths of the call.
+spt_status make_job(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* prm, float* d_rgb32, double* d_rgb64,
+                    float* d_path_rgb, Job& jb, uint64_t& total) {
+    jb = Job{};
+    jb.cam = make_camera(*cam);
+    jb.sampler = prm->pixel_sampler;
+    if (jb.sampler > SPT_PIXEL_UNIFORM) return set_error(ctx, SPT_ERR_INVALID, "unknown pixel sampler");
     if (prm->spp == 0) return set_error(ctx, SPT_ERR_INVALID, "spp must be > 0");
-    if (a.sampler == SPT_PIXEL_UNIFORM) {
-        a.n_sub = 1;
-        a.spp = prm->spp;
-        a.w_sub = 1.0f;
+    if (jb.sampler == SPT_PIXEL_UNIFORM) {
+        jb.n_sub = 1;
+        jb.spp = prm->spp;
+        jb.w_sub = 1.0f;
     } else {  // n = ceil(sqrt(spp)), spp := n^2 (jittered_pixel_sampler.cpp:6-8)
-        a.n_sub = (uint32_t)std::ceil(std::sqrt((double)prm->spp));
-        a.spp = a.n_sub * a.n_sub;
-        a.w_sub = 1.0f / a.n_sub;
+        jb.n_sub = (uint32_t)std::ceil(std::sqrt((double)prm->spp));
+        jb.spp = jb.n_sub * jb.n_sub;
+        jb.w_sub = 1.0f / jb.n_sub;
     }
-    a.spp_begin = prm->spp_begin;
-    a.spp_count = prm->spp_count ? prm->spp_count : a.spp - prm->spp_begin;
-    if ((uint64_t)a.spp_begin + a.spp_count > a.spp || a.spp_count == 0)
+    jb.spp_begin = prm->spp_begin;
+    jb.spp_count = prm->spp_count ? prm->spp_count : jb.spp - std::min(prm->spp_begin, jb.spp);
+    if ((uint64_t)jb.spp_begin + jb.spp_count > jb.spp || jb.spp_count == 0)
         return set_error(ctx, SPT_ERR_INVALID, "sample range outside [0, spp)");
-    a.vt = prm->viewport[0];
-    a.vl = prm->viewport[1];
-    a.vw = prm->viewport[2];
-    a.vh = prm->viewport[3];
-    if (a.vw == 0) {
-        a.vt = a.vl = 0;
-        a.vw = cam->width;
-        a.vh = cam->height;
+    jb.vt = prm->viewport[0];
+    jb.vl = prm->viewport[1];
+    jb.vw = prm->viewport[2];
+    jb.vh = prm->viewport[3];
+    if (jb.vw == 0) {
+        jb.vt = jb.vl = 0;
+        jb.vw = cam->width;
+        jb.vh = cam->height;
     }
-    if ((uint64_t)a.vl + a.vw > cam->width || (uint64_t)a.vt + a.vh > cam->height || a.vh == 0)
+    if ((uint64_t)jb.vl + jb.vw > cam->width || (uint64_t)jb.vt + jb.vh > cam->height || jb.vh == 0)
         return set_error(ctx, SPT_ERR_INVALID, "viewport outside the image");
-    a.max_depth = prm->max_depth;
-    a.rr_depth = prm->rr_depth;
-    if (a.max_depth < 0 && a.rr_depth < 0)
+    jb.max_depth = prm->max_depth;
+    jb.rr_depth = prm->rr_depth;
+    if (jb.max_depth < 0 && jb.rr_depth < 0)
         return set_error(ctx, SPT_ERR_INVALID, "need max_depth >= 0 or rr_depth >= 0 (paths must terminate)");
-    a.seed = prm->seed;
-    for (int i = 0; i < 3; ++i) a.bg[i] = prm->background[i];
-    a.flags = prm->flags;
-    a.film32 = (prm->flags & SPT_RENDER_FILM32) ? d_rgb32 : nullptr;
-    a.film64 = (prm->flags & SPT_RENDER_FILM64) ? d_rgb64 : nullptr;
-    a.path_rgb = (prm->flags & SPT_RENDER_PER_PATH) ? d_path_rgb : nullptr;
+    jb.seed = prm->seed;
+    for (int i = 0; i < 3; ++i) jb.bg[i] = prm->background[i];
+    jb.film32 = (prm->flags & SPT_RENDER_FILM32) ? d_rgb32 : nullptr;
+    jb.film64 = (prm->flags & SPT_RENDER_FILM64) ? d_rgb64 : nullptr;
+    jb.path_rgb = (prm->flags & SPT_RENDER_PER_PATH) ? d_path_rgb : nullptr;
     if ((prm->flags & SPT_RENDER_FILM32) && !d_rgb32) return set_error(ctx, SPT_ERR_INVALID, "FILM32 without buffer");
     if ((prm->flags & SPT_RENDER_FILM64) && !d_rgb64) return set_error(ctx, SPT_ERR_INVALID, "FILM64 without buffer");
     if ((prm->flags & SPT_RENDER_PER_PATH) && !d_path_rgb)
         return set_error(ctx, SPT_ERR_INVALID, "PER_PATH without buffer");
-    a.guided = (prm->flags & SPT_RENDER_GUIDED) ? 1u : 0u;
-    a.train = (a.guided && (prm->flags & SPT_RENDER_GUIDE_TRAIN)) ? 1u : 0u;
-    if ((prm->flags & SPT_RENDER_GUIDE_TRAIN) && !a.guided)
-        return set_error(ctx, SPT_ERR_INVALID, "GUIDE_TRAIN needs GUIDED");
-    if (a.guided && !ctx->guide.ready) return set_error(ctx, SPT_ERR_STATE, "guided rendering needs spt_guide_init");
-    a.guidable = ctx->d_guidable;
-    a.bsdf_frac = ctx->guide.bsdf_fraction;
-    a.records = ctx->guide.d_records;
-    a.rec_cap = ctx->guide.records_cap;
     if (prm->rng == SPT_RNG_STREAM) {
         if (!prm->d_stream || prm->stream_stride == 0)
             return set_error(ctx, SPT_ERR_INVALID, "replay stream missing");
-        a.stream = prm->d_stream;
-        a.stride = prm->stream_stride;
+        jb.stream = prm->d_stream;
+        jb.stride = prm->stream_stride;
     } else if (prm->rng != SPT_RNG_COUNTER) {
         return set_error(ctx, SPT_ERR_INVALID, "unknown rng mode");
     }
-    const uint64_t total = (uint64_t)a.vw * a.vh * a.spp_count;
-    const uint32_t keys = ctx->n_mats + 1;
-    a.keys = (!(prm->flags & SPT_RENDER_NO_SORT) && keys <= kMaxSortKeys) ? keys : 0u;
+    total = (uint64_t)jb.vw * jb.vh * jb.spp_count;
+    jb.n_paths = total;
+    return SPT_OK;
+}
 
-    // path slots: enough for the whole call, at most 4M in flight
-    const uint64_t cap = std::min<uint64_t>(total, 1u << 22);
-    const size_t bytes = (size_t)cap * (2 * kBufBytes + kWorkBytes + 4 * 8) +
-                         (size_t)kMaxSortKeys * kSortBlocks * 4 + 4096;
+namespace {
+
+// Runs `total` paths through the wavefront: generation in chunks of the path
+// capacity, then bounces until no path is live (or cap_depth bounces).
+template <bool MULTI>
+spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_depth, const spt_tuning& tn,
+                     spt_render_stats& stats) {
+    const uint64_t cap = std::min<uint64_t>(total, tn.path_capacity);
+    const size_t bytes = (size_t)cap * (2 * kBufBytes + kWorkBytes + 4 * 8) + (size_t)kMaxSortKeys * kSortBlocks * 4 +
+                         4096;
     if (bytes > ctx->paths_bytes) {
         (void)hipStreamSynchronize(ctx->stream);
         if (ctx->d_paths) (void)hipFree(ctx->d_paths);
@@ -629,18 +689,19 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     }
 
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_extend<true>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_extend<true, MULTI>),
                                                      kBlock, 0) != hipSuccess ||
         per_cu <= 0)
         per_cu = 4;
+    if (tn.extend_blocks_per_cu) per_cu = std::min<int>(per_cu, (int)tn.extend_blocks_per_cu);
     const uint32_t ext_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)(per_cu * ctx->num_cus));
-    const uint32_t shade_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * 8);
+    const uint32_t shade_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * tn.shade_blocks_per_cu);
     spt_status s = ensure_spill(ctx, (size_t)ext_grid * kBlock *
-                                         (ctx->stack_cap > kLdsStack ? ctx->stack_cap - kLdsStack : 1));
+                                         (a.sc.stack_cap > kLdsStack ? a.sc.stack_cap - kLdsStack : 1));
     if (s != SPT_OK) return s;
     hipStream_t st = ctx->stream;
-    const bool timed = (prm->flags & SPT_RENDER_TIME_KERNELS) != 0;
-    const bool fetch_stats = (prm->flags & SPT_RENDER_FETCH_STATS) != 0;
+    const bool timed = (a.flags & SPT_RENDER_TIME_KERNELS) != 0;
+    const bool fetch_stats = (a.flags & SPT_RENDER_FETCH_STATS) != 0;
     s = check_hip(ctx, hipMemsetAsync(w.cnt, 0, 256, st), "counters");
     if (s != SPT_OK) return s;
     if (a.train) {  // records continue after those collected since the last training step
@@ -664,72 +725,133 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
         s = check_hip(ctx, hipMemsetAsync(ctx->d_flags + 4, 0, 16, st), "fetch counters");
         if (s != SPT_OK) return s;
     }
-    // bounces are issued without waiting; the live count is read every kCheck bounces
-    constexpr uint32_t kCheck = 4;
-    spt_render_stats stats{};
-    stats.paths = total;
+    std::vector<hipEvent_t> gev;  // guide inference launches (TIME_KERNELS)
+    // One bounce over buffer (b & 1) into the other buffer.
+    auto bounce = [&](uint32_t b) -> spt_status {
+        PathBuf& A = buf[b & 1];
+        PathBuf& B = buf[(b + 1) & 1];
+        uint32_t* cntA = w.cnt + (b & 1);
+        uint32_t* cntB = w.cnt + ((b + 1) & 1);
+        if (timed) {
+            while (ctx->tev.size() < 2 * (stats.extend_launches + 1)) {
+                hipEvent_t e;
+                spt_status es = check_hip(ctx, hipEventCreate(&e), "event");
+                if (es != SPT_OK) return es;
+                ctx->tev.push_back(e);
+            }
+            (void)hipEventRecord(ctx->tev[2 * stats.extend_launches], st);
+        }
+        if (fetch_stats)
+            hipLaunchKernelGGL((k_extend<true, MULTI>), dim3(ext_grid), dim3(kBlock), 0, st, a, A, w, cntA,
+                               ctx->d_spill, ctx->d_flags);
+        else
+            hipLaunchKernelGGL((k_extend<false, MULTI>), dim3(ext_grid), dim3(kBlock), 0, st, a, A, w, cntA,
+                               ctx->d_spill, ctx->d_flags);
+        if (timed) {
+            (void)hipEventRecord(ctx->tev[2 * stats.extend_launches + 1], st);
+            ++stats.extend_launches;
+        }
+        if (a.keys) {
+            hipLaunchKernelGGL(k_hist, dim3(kSortBlocks), dim3(kBlock), 0, st, a, w, cntA);
+            hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, a, w);
+            hipLaunchKernelGGL(k_scatter, dim3(kSortBlocks), dim3(kBlock), 0, st, a, w, cntA);
+        }
+        if (a.guided) {
+            if (timed) {
+                hipEvent_t e0, e1;
+                if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+                    gev.push_back(e0);
+                    gev.push_back(e1);
+                    (void)hipEventRecord(e0, st);
+                }
+            }
+            hipLaunchKernelGGL(k_guide_paths, dim3((uint32_t)ctx->num_cus * 2), dim3(kBlock), guide_paths_lds_bytes(),
+                               st, a, A, w, cntA, ctx->guide.d_theta, ctx->guide.d_wbf, guide_box(ctx));
+            if (timed && !gev.empty()) (void)hipEventRecord(gev.back(), st);
+        }
+        hipLaunchKernelGGL((k_shade<MULTI>), dim3(shade_grid), dim3(kBlock), 0, st, a, A, B, w, cntA, cntB);
+        hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, st, cntA, w.cnt + 8);
+        return check_hip(ctx, hipGetLastError(), "bounce launch");
+    };
+    // Graph mode: two bounces (one per buffer parity) captured once and
+    // replayed; every kernel reads its live count from device memory, so the
+    // same graph serves every bounce. No memset nodes are recorded.
+    const bool graph_on = tn.use_graph == 1 && !timed && !fetch_stats;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto drop_graph = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    };
+    if (graph_on) {
+        s = check_hip(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
+        if (s != SPT_OK) return s;
+        spt_status s0 = bounce(0);
+        spt_status s1 = bounce(1);
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (s0 != SPT_OK || s1 != SPT_OK) {
+            drop_graph();
+            return s0 != SPT_OK ? s0 : s1;
+        }
+        s = check_hip(ctx, e, "graph capture end");
+        if (s == SPT_OK) s = check_hip(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+        if (s != SPT_OK) {
+            drop_graph();
+            return s;
+        }
+    }
+    const uint32_t check = graph_on ? std::max<uint32_t>(2, (tn.check_every + 1) & ~1u) : tn.check_every;
     for (uint64_t done = 0; done < total; done += cap) {
         const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - done);
-        hipLaunchKernelGGL(k_generate, dim3(blocks_for(n)), dim3(kBlock), 0, st, a, buf[0], w, done, n);
+        hipLaunchKernelGGL((k_generate<MULTI>), dim3(blocks_for(n)), dim3(kBlock), 0, st, a, buf[0], w, done, n);
         s = check_hip(ctx, hipGetLastError(), "generate launch");
-        if (s != SPT_OK) return s;
-        if (a.max_depth == 0) continue;
+        if (s != SPT_OK) break;
+        if (!MULTI && a.job.max_depth == 0) continue;
         uint32_t b = 0;
         while (true) {
-            PathBuf& A = buf[b & 1];
-            PathBuf& B = buf[(b + 1) & 1];
-            uint32_t* cntA = w.cnt + (b & 1);
-            uint32_t* cntB = w.cnt + ((b + 1) & 1);
-            if (timed) {
-                while (ctx->tev.size() < 2 * (stats.extend_launches + 1)) {
-                    hipEvent_t e;
-                    s = check_hip(ctx, hipEventCreate(&e), "event");
-                    if (s != SPT_OK) return s;
-                    ctx->tev.push_back(e);
-                }
-                (void)hipEventRecord(ctx->tev[2 * stats.extend_launches], st);
+            if (graph_on) {
+                s = check_hip(ctx, hipGraphLaunch(exec, st), "graph launch");
+                b += 2;
+            } else {
+                s = bounce(b);
+                ++b;
             }
-            if (fetch_stats)
-                hipLaunchKernelGGL(k_extend<true>, dim3(ext_grid), dim3(kBlock), 0, st, a, A, w, cntA, ctx->d_spill,
-                                   ctx->d_flags);
-            else
-                hipLaunchKernelGGL(k_extend<false>, dim3(ext_grid), dim3(kBlock), 0, st, a, A, w, cntA, ctx->d_spill,
-                                   ctx->d_flags);
-            if (timed) {
-                (void)hipEventRecord(ctx->tev[2 * stats.extend_launches + 1], st);
-                ++stats.extend_launches;
-            }
-            if (a.keys) {
-                hipLaunchKernelGGL(k_hist, dim3(kSortBlocks), dim3(kBlock), 0, st, a, w, cntA);
-                hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, a, w);
-                hipLaunchKernelGGL(k_scatter, dim3(kSortBlocks), dim3(kBlock), 0, st, a, w, cntA);
-            }
-            if (a.guided)
-                hipLaunchKernelGGL(k_guide_paths, dim3((uint32_t)ctx->num_cus * 2), dim3(kBlock),
-                                   guide_paths_lds_bytes(), st, a, A, w, cntA, ctx->guide.d_theta, ctx->guide.d_wbf,
-                                   guide_box(ctx));
-            hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(kBlock), 0, st, a, A, B, w, cntA, cntB);
-            hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, st, cntA, w.cnt + 8);
-            s = check_hip(ctx, hipGetLastError(), "bounce launch");
-            if (s != SPT_OK) return s;
-            ++b;
-            const bool capped = (a.max_depth >= 0) && (b >= (uint32_t)a.max_depth);
-            if (capped) break;
-            if (b % kCheck == 0) {
+            if (s != SPT_OK) break;
+            if (cap_depth >= 0 && b >= (uint32_t)cap_depth) break;
+            if (b % check == 0) {
                 uint32_t live = 0;
                 s = check_hip(ctx, hipMemcpyAsync(&live, w.cnt + (b & 1), 4, hipMemcpyDeviceToHost, st), "count");
                 if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "bounce");
-                if (s != SPT_OK) return s;
-                if (live == 0) break;
+                if (s != SPT_OK || live == 0) break;
             }
         }
+        if (s != SPT_OK) break;
     }
-    uint32_t cnt[16];
+    if (s != SPT_OK) {
+        (void)hipStreamSynchronize(st);
+        drop_graph();
+        return s;
+    }
+    uint32_t cnt[20];  // [16]: guide records written, [17]: candidate vertices
     s = check_hip(ctx, hipMemcpyAsync(cnt, w.cnt, sizeof(cnt), hipMemcpyDeviceToHost, st), "counters");
     if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "render");
+    drop_graph();
     if (s != SPT_OK) return s;
     stats.stream_overflows = cnt[2];
-    if (a.train) ctx->guide.collected = std::min<uint64_t>(cnt[16], a.rec_cap);
+    if (a.train) {
+        GuideState& gs = ctx->guide;
+        gs.collected = std::min<uint64_t>(cnt[16], a.rec_cap);
+        gs.candidates = cnt[17];
+        // next pass: record about one batch worth of candidates
+        gs.rec_stride = (uint32_t)std::max<uint64_t>(1, (cnt[17] + gs.batch - 1) / std::max<uint32_t>(gs.batch, 1));
+    }
+    for (size_t i = 0; i + 1 < gev.size(); i += 2) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, gev[i], gev[i + 1]) == hipSuccess) stats.guide_ms += ms;
+    }
+    for (hipEvent_t e : gev) (void)hipEventDestroy(e);
     stats.rays = (uint64_t)cnt[10] | ((uint64_t)cnt[11] << 32);
     stats.bounces = (uint64_t)cnt[12] | ((uint64_t)cnt[13] << 32);
     if (timed) {
@@ -738,8 +860,6 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
             if (hipEventElapsedTime(&ms, ctx->tev[2 * i], ctx->tev[2 * i + 1]) == hipSuccess) stats.extend_ms += ms;
         }
     }
-    stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    ctx->stats = stats;
     uint32_t flags = 0;
     s = check_hip(ctx, hipMemcpy(&flags, ctx->d_flags, 4, hipMemcpyDeviceToHost), "flags");
     if (s != SPT_OK) return s;
@@ -747,14 +867,174 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
         unsigned long long f[2] = {0, 0};
         s = check_hip(ctx, hipMemcpy(f, ctx->d_flags + 4, 16, hipMemcpyDeviceToHost), "fetch counters");
         if (s != SPT_OK) return s;
-        ctx->stats.node_fetches = f[0];
-        ctx->stats.tri_fetches = f[1];
+        stats.node_fetches = f[0];
+        stats.tri_fetches = f[1];
     }
     if (flags & 1u) {
         (void)hipMemset(ctx->d_flags, 0, 4);
         return set_error(ctx, SPT_ERR_OVERFLOW, "traversal stack overflow");
     }
     if (stats.stream_overflows) return set_error(ctx, SPT_ERR_OVERFLOW, "replay stream exhausted");
+    return SPT_OK;
+}
+
+}  // namespace
+
+spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* prm, float* d_rgb32,
+                         double* d_rgb64, float* d_path_rgb) {
+    auto t0 = std::chrono::steady_clock::now();
+    RenderArgs a{};
+    a.sc = ctx->scene();
+    uint64_t total = 0;
+    spt_status s = make_job(ctx, cam, prm, d_rgb32, d_rgb64, d_path_rgb, a.job, total);
+    if (s != SPT_OK) return s;
+    for (int i = 0; i < 3; ++i) {
+        a.job.rmin[i] = ctx->root_box[i];
+        a.job.rmax[i] = ctx->root_box[3 + i];
+    }
+    a.job.root_leaf = ctx->root_leaf;
+    a.flags = prm->flags;
+    a.guided = (prm->flags & SPT_RENDER_GUIDED) ? 1u : 0u;
+    a.train = (a.guided && (prm->flags & SPT_RENDER_GUIDE_TRAIN)) ? 1u : 0u;
+    if ((prm->flags & SPT_RENDER_GUIDE_TRAIN) && !a.guided)
+        return set_error(ctx, SPT_ERR_INVALID, "GUIDE_TRAIN needs GUIDED");
+    if (a.guided && !ctx->guide.ready) return set_error(ctx, SPT_ERR_STATE, "guided rendering needs spt_guide_init");
+    a.guidable = ctx->d_guidable;
+    a.bsdf_frac = ctx->guide.bsdf_fraction;
+    a.records = ctx->guide.d_records;
+    a.rec_cap = ctx->guide.records_cap;
+    a.rec_stride = ctx->guide.rec_stride;
+    const uint32_t keys = ctx->n_mats + 1;
+    a.keys = (!(prm->flags & SPT_RENDER_NO_SORT) && keys <= kMaxSortKeys) ? keys : 0u;
+    spt_render_stats stats{};
+    stats.paths = total;
+    s = run_paths<false>(ctx, a, total, a.job.max_depth, resolve_tuning(ctx->tuning), stats);
+    stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->stats = stats;
+    return s;
+}
+
+spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div) {
+    auto t0 = std::chrono::steady_clock::now();
+    const spt_tuning tn = resolve_tuning(ctx->tuning);
+    const uint32_t nj = (uint32_t)b.jobs.size();
+    std::vector<Job> jobs = b.jobs;
+    int32_t cap_depth = 0;
+    for (Job& jb : jobs) {
+        if (sample_div > 1) {  // tuning trial: a share of the samples, outputs untouched
+            jb.spp_count = std::max<uint32_t>(1, jb.spp_count / sample_div);
+            jb.film32 = nullptr;
+            jb.film64 = nullptr;
+            jb.path_rgb = nullptr;
+        }
+        jb.n_paths = (uint64_t)jb.vw * jb.vh * jb.spp_count;
+        if (cap_depth >= 0) cap_depth = (jb.max_depth < 0) ? -1 : std::max(cap_depth, jb.max_depth);
+    }
+    // generation tiles, taken round-robin from the jobs; within a job samples
+    // advance slowest, so every pixel gets its first samples early
+    std::vector<std::vector<BatchTile>> per(nj);
+    for (uint32_t j = 0; j < nj; ++j) {
+        const Job& jb = jobs[j];
+        const uint32_t npix = jb.vw * jb.vh;
+        for (uint32_t s0 = 0; s0 < jb.spp_count; s0 += tn.tile_samples)
+            for (uint32_t p0 = 0; p0 < npix; p0 += tn.tile_pixels) {
+                BatchTile t{};
+                t.job = j;
+                t.p0 = p0;
+                t.np = std::min(tn.tile_pixels, npix - p0);
+                t.s0 = jb.spp_begin + s0;
+                t.ns = std::min(tn.tile_samples, jb.spp_count - s0);
+                per[j].push_back(t);
+            }
+    }
+    std::vector<BatchTile> tiles;
+    std::vector<uint64_t> start;
+    uint64_t total = 0;
+    for (size_t r = 0;; ++r) {
+        bool any = false;
+        for (uint32_t j = 0; j < nj; ++j) {
+            if (r >= per[j].size()) continue;
+            any = true;
+            tiles.push_back(per[j][r]);
+            start.push_back(total);
+            total += (uint64_t)per[j][r].np * per[j][r].ns;
+        }
+        if (!any) break;
+    }
+    start.push_back(total);
+    if (tiles.empty() || tiles.size() >= (1u << 31)) return set_error(ctx, SPT_ERR_INVALID, "empty or oversized batch");
+    if (tiles.size() > b.tiles_cap) {
+        if (b.d_tiles) (void)hipFree(b.d_tiles);
+        if (b.d_tile_start) (void)hipFree(b.d_tile_start);
+        b.d_tiles = nullptr;
+        b.d_tile_start = nullptr;
+        b.tiles_cap = 0;
+        spt_status s = check_hip(ctx, hipMalloc(&b.d_tiles, tiles.size() * sizeof(BatchTile)), "batch tiles");
+        if (s == SPT_OK)
+            s = check_hip(ctx, hipMalloc(&b.d_tile_start, (tiles.size() + 1) * sizeof(uint64_t)), "batch tiles");
+        if (s != SPT_OK) return s;
+        b.tiles_cap = tiles.size();
+    }
+    hipStream_t st = ctx->stream;
+    spt_status s = check_hip(ctx, hipMemcpyAsync(b.d_tiles, tiles.data(), tiles.size() * sizeof(BatchTile),
+                                                 hipMemcpyHostToDevice, st), "tiles upload");
+    if (s == SPT_OK)
+        s = check_hip(ctx, hipMemcpyAsync(b.d_tile_start, start.data(), start.size() * sizeof(uint64_t),
+                                          hipMemcpyHostToDevice, st), "tiles upload");
+    if (s == SPT_OK)
+        s = check_hip(ctx, hipMemcpyAsync(b.d_jobs, jobs.data(), nj * sizeof(Job), hipMemcpyHostToDevice, st),
+                      "jobs upload");
+    if (s == SPT_OK) s = check_hip(ctx, hipMemsetAsync(b.d_done, 0, nj * sizeof(uint32_t), st), "job counters");
+    if (s == SPT_OK) s = check_hip(ctx, hipMemsetAsync(b.d_clock, 0, (nj + 1) * sizeof(uint64_t), st), "job clocks");
+    if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "batch upload");  // host vectors go out of scope
+    if (s != SPT_OK) return s;
+
+    RenderArgs a{};
+    a.sc.nodes = b.d_nodes;
+    a.sc.tris = b.d_tris;
+    a.sc.shading = b.d_shading;
+    a.sc.mats = b.d_mats;
+    a.sc.n_tris = b.n_tris;
+    a.sc.n_nodes = b.n_nodes;
+    a.sc.n_mats = b.n_mats;
+    a.sc.stack_cap = b.stack_cap;
+    a.jobs = b.d_jobs;
+    a.tiles = b.d_tiles;
+    a.tile_start = b.d_tile_start;
+    a.n_tiles = (uint32_t)tiles.size();
+    a.job_done = b.d_done;
+    a.job_clock = b.d_clock;
+    a.flags = 0;
+    const uint32_t keys = b.n_mats + 1;
+    a.keys = keys <= kMaxSortKeys ? keys : 0u;
+    spt_render_stats stats{};
+    stats.paths = total;
+    s = run_paths<true>(ctx, a, total, cap_depth, tn, stats);
+    if (s != SPT_OK) return s;
+    std::vector<uint64_t> clk(nj + 1, 0);
+    std::vector<uint32_t> done(nj, 0);
+    s = check_hip(ctx, hipMemcpy(clk.data(), b.d_clock, clk.size() * sizeof(uint64_t), hipMemcpyDeviceToHost),
+                  "job clocks");
+    if (s == SPT_OK)
+        s = check_hip(ctx, hipMemcpy(done.data(), b.d_done, done.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
+                      "job counters");
+    if (s != SPT_OK) return s;
+    b.scene_ms.assign(nj, 0.0);
+    for (uint32_t j = 0; j < nj; ++j) {
+        if (done[j] != jobs[j].n_paths) return set_error(ctx, SPT_ERR_DEVICE, "a scene did not finish all its paths");
+        b.scene_ms[j] = (double)(clk[1 + j] - clk[0]) / (double)ctx->wall_clock_khz;
+    }
+    b.stats = spt_batch_stats{};
+    b.stats.scenes = nj;
+    b.stats.paths = total;
+    b.stats.rays = stats.rays;
+    b.stats.bounces = stats.bounces;
+    b.stats.tiles = tiles.size();
+    b.stats.triangles = b.n_tris;
+    b.stats.device_bytes = b.device_bytes;
+    b.stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->stats = stats;
+    ctx->stats.ms = b.stats.ms;
     return SPT_OK;
 }
 

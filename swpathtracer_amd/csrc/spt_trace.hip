@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_p(DevScene sc, const float4* _
     bool overflow = false;
     trace_persistent<FETCH>(
         sc, n, flags + 8, st, overflow,
-        [&](uint32_t i, V3& o, V3& d) {
+        [&](uint32_t i, V3& o, V3& d, Root&) {
             const float4 ro = rays[2 * (size_t)i], rd = rays[2 * (size_t)i + 1];
             o = V3{ro.x, ro.y, ro.z};
             d = V3{rd.x, rd.y, rd.z};

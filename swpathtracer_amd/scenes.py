@@ -320,3 +320,44 @@ def furnace(subdiv: int = 3, albedo=(0.999, 0.999, 0.999), layer=None, width=64,
     sc.integrator = api.Integrator(spp=spp, pixel_sampler="jittered", termination="russian-roulette",
                                    path_length=3, seed=7)
     return sc
+
+
+def batch_scenes(n: int = 64, seed: int = 4, min_tris: int = 1000, max_tris: int = 500_000, min_side: int = 128,
+                 max_side: int = 1024, spp_choices=(16, 64, 256), max_depth: int = 16) -> List[SceneData]:
+    """Independent scenes for batched rendering: each a Lambertian box with a
+    ceiling light around a seeded icosphere or Perlin-displaced sphere whose
+    triangle count is log-uniform in [min_tris, max_tris]; square resolution
+    uniform in [min_side, max_side]; spp from spp_choices; object material
+    Lambertian, GGX (alpha 0.05-0.5, gold) or smooth dielectric (IOR 1.5);
+    Russian roulette from depth 3, max depth max_depth."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        target = float(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris))))
+        kind = rng.integers(3)
+        if kind == 0:
+            obj = api.material([api.lambertian(tuple(rng.uniform(0.2, 0.9, 3)))])
+        elif kind == 1:
+            obj = api.material([api.ggx_conductor(float(rng.uniform(0.05, 0.5)))])
+        else:
+            obj = api.material([api.smooth_dielectric(1.5)])
+        mats = _box_materials() + [obj]
+        parts = box_walls((-1, -1, -1), (1, 1, 1), 0, 1, 2)
+        parts.append(quad((-0.3, 0.999, -0.3), (0.3, 0.999, -0.3), (0.3, 0.999, 0.3), (-0.3, 0.999, 0.3),
+                          (0, -1, 0), 3))
+        if rng.integers(2) == 0:
+            sub = int(np.clip(np.round(np.log(max(target, 20) / 20.0) / np.log(4.0)), 1, 7))
+            center = (rng.uniform(-0.2, 0.2), -0.45, rng.uniform(-0.2, 0.2))
+            mesh = icosphere(sub, center, 0.45, random_rotation(rng), mat=4)
+        else:
+            rings = max(4, int(round(np.sqrt(target / 4.0))))
+            mesh = displaced_sphere(rings, 2 * rings, 0.1, 4.0, int(rng.integers(1 << 30)), center=(0.0, -0.45, 0.0),
+                                    scale=0.45, mat=4)
+        sc = concat(parts + [mesh], mats, name=f"batch{i}_{mesh.n_tris}")
+        side = int(rng.integers(min_side, max_side + 1))
+        spp = int(spp_choices[rng.integers(len(spp_choices))])
+        sc.camera = api.pinhole(side, side, position=(0, 0, 3.6), look_at=(0, 0, 0), fov=40.0)
+        sc.integrator = api.Integrator(spp=spp, pixel_sampler="jittered", termination="russian-roulette",
+                                       path_length=3, max_depth=max_depth, seed=int(rng.integers(1 << 31)))
+        out.append(sc)
+    return out

@@ -139,9 +139,18 @@ def test_loss_scale_backoff(device, box_scene):
     the scale until the gradient is finite again."""
     sc, bmin, bext = box_scene
     device.guide_init(seed=5, loss_scale=3.0e38)
-    theta = _theta(14, out_scale=3.0)
+    # eight equal lobes around +z with kappa 1000 and one record pointing to -z:
+    # d loss / d log kappa = -W r (1 - 2k/(e^2k - 1) + k (mu.d - 1)) ~ 250 per lobe,
+    # so the scaled gradient of a one-record batch is certain to overflow
+    th = G.unpack(_theta(14).astype(np.float64).copy())
+    th["W4"][:] = 0.0
+    for k in range(G.LOBES):
+        th["b4"][5 * k:5 * k + 5] = [0.0, 0.0, 1.0, np.log(1000.0), 0.0]
+    theta = G.pack(th).astype(np.float32)
     device.guide_set_params(theta)
-    rec = _records(np.random.default_rng(5), 64, bmin, bext)
+    rec = _records(np.random.default_rng(5), 1, bmin, bext)
+    rec[0, 9:12] = [0.0, 0.0, -1.0]
+    rec[0, 12] = 1.0
     scale = 3.0e38
     for _ in range(140):
         device.guide_train(rec)
