@@ -208,6 +208,43 @@ def guide_experiment(args):
     return res
 
 
+def batch_bench(args):
+    """64 independent scenes (box + seeded icosphere or displaced sphere,
+    1k-500k triangles log-uniform, 128^2-1024^2, 16/64/256 spp, Lambertian /
+    GGX / dielectric objects) rendered by one wavefront after autotuning tile
+    and queue sizes on a 1/64 sample of every scene's samples."""
+    from swpathtracer_amd import api, scenes
+
+    t0 = time.perf_counter()
+    scs = scenes.batch_scenes(n=64, seed=4)
+    log(f"batch: {len(scs)} scenes, {sum(s.n_tris for s in scs)} triangles ({time.perf_counter() - t0:.1f} s)")
+    dev = api.Device(0)
+    t0 = time.perf_counter()
+    b = api.Batch(dev, scs, n_threads=os.cpu_count() or 8)
+    log(f"batch trees built and uploaded ({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    tuning = b.autotune(sample_div=64)
+    log(f"autotuned in {time.perf_counter() - t0:.1f} s: {tuning}")
+    b.clear()
+    dev.sync()
+    t0 = time.perf_counter()
+    b.render()
+    dev.sync()
+    wall = time.perf_counter() - t0
+    st, ms = b.stats()
+    paths = st["paths"]
+    res = {"scenes": len(scs), "triangles": int(st["triangles"]), "paths": int(paths),
+           "wall_ms": round(wall * 1e3, 2), "mpaths_s": round(paths / wall / 1e6, 3),
+           "grays_s": round(st["rays"] / wall / 1e9, 4),
+           "scene_ms_min": round(float(ms.min()), 2), "scene_ms_median": round(float(np.median(ms)), 2),
+           "scene_ms_max": round(float(ms.max()), 2), "tuning": tuning,
+           "device_mb": round(st["device_bytes"] / 2 ** 20, 1)}
+    log(f"batch: {res}")
+    b.close()
+    dev.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +257,8 @@ def main():
     ap.add_argument("--segments", type=int, default=1024)
     ap.add_argument("--guide", action="store_true", help="also run the guided vs plain equal-RMSE experiment")
     ap.add_argument("--guide-only", action="store_true")
+    ap.add_argument("--batch", action="store_true", help="also run the 64-scene batched render")
+    ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--guide-full", action="store_true", help="keep rendering after equal RMSE (full curve)")
     ap.add_argument("--guide-res", type=int, nargs=2, default=(800, 600))
     ap.add_argument("--oracle-spp", type=int, default=16384)
@@ -250,6 +289,9 @@ def main():
 
     if args.guide_only:
         print(json.dumps({"guide_experiment": guide_experiment(args)}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": batch_bench(args)}), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -318,6 +360,8 @@ def main():
         extra["closest_hit_incoherent"] = {k: round(v, 4) if isinstance(v, float) else v for k, v in ch.items()}
     if args.guide:
         extra["guide_experiment"] = guide_experiment(args)
+    if args.batch:
+        extra["batch"] = batch_bench(args)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sc)

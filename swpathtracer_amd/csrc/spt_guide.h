@@ -45,9 +45,10 @@ __device__ inline void encode(V3 p, V3 n, V3 wo, const float* bmin, const float*
     int o = 3;
     for (int k = 0; k < 3; ++k) {
         for (int c = 0; c < 3; ++c) {
-            const float a = 2.0f * kPi * (float)(1 << k) * x[c];
-            f[o++] = sinf(a);
-            f[o++] = cosf(a);
+            // v_sin / v_cos take the angle in revolutions: sin(2 pi 2^k x)
+            const float a = (float)(1 << k) * x[c];
+            f[o++] = __builtin_amdgcn_sinf(a);
+            f[o++] = __builtin_amdgcn_cosf(a);
         }
     }
     f[21] = n.x;
@@ -59,7 +60,7 @@ __device__ inline void encode(V3 p, V3 n, V3 wo, const float* bmin, const float*
     const float c = 0.5f * (dot(n, wo) + 1.0f);
     for (int b = 0; b < 4; ++b) {
         const float dc = c - (b + 0.5f) * 0.25f;
-        f[27 + b] = expf(-dc * dc / (2.0f * 0.0625f));
+        f[27 + b] = __expf(-dc * dc * 8.0f);  // sigma 1/4: exp(-dc^2 / (2 sigma^2))
     }
     f[31] = 1.0f;
 }
@@ -157,6 +158,109 @@ __device__ inline void mlp_tile_fwd(const SharedWeights& s, WaveTiles& t) {
     dense_fwd<kHid, kHid, true>(t.h1, s.w2, s.b2, t.h2, nullptr);
     dense_fwd<kHid, kHid, true>(t.h2, s.w3, s.b3, t.h3, nullptr);
     dense_fwd<kHid, kOutP, false>(t.h3, s.w4, s.b4, nullptr, t.y);
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident forward pass ("column" form): the layer is computed as
+// out^T = W * in^T, weights as the MFMA A operand (from LDS) and 16 samples as
+// the B operand. A lane (r, g) then holds features 16 t + 4 g + i (i = 0..3)
+// of sample r in each output tile t, which is exactly the B-operand layout of
+// the next layer's k-tile t: activations never leave registers.
+// ---------------------------------------------------------------------------
+__device__ inline unsigned short f2bf_relu(float v) { return f2bf(fmaxf(v, 0.0f)); }
+
+__device__ inline bf16x4 pack_relu(f32x4 v) {
+    bf16x4 r;
+    r[0] = (short)f2bf_relu(v[0]);
+    r[1] = (short)f2bf_relu(v[1]);
+    r[2] = (short)f2bf_relu(v[2]);
+    r[3] = (short)f2bf_relu(v[3]);
+    return r;
+}
+
+__device__ inline f32x4 bias4(const float* b) { return f32x4{b[0], b[1], b[2], b[3]}; }
+
+// x[kt]: features 16 kt + 4 g + j of sample r (bf16); y[t]: raw outputs
+// 16 t + 4 g + i of sample r (f32), t = 0..2 (outputs 40..47 are padding).
+__device__ inline void mlp_cols_fwd(const SharedWeights& s, const bf16x4 x[2], f32x4 y[3]) {
+    const int l = __lane_id(), r = l & 15, g = l >> 4;
+    bf16x4 ha[4], hb[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 acc = bias4(s.b1 + 16 * t + 4 * g);
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(s.w1 + (16 * t + r) * kEnc + 16 * k + 4 * g), x[k], acc,
+                                                            0, 0, 0);
+        ha[t] = pack_relu(acc);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 acc = bias4(s.b2 + 16 * t + 4 * g);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(s.w2 + (16 * t + r) * kHid + 16 * k + 4 * g), ha[k], acc,
+                                                            0, 0, 0);
+        hb[t] = pack_relu(acc);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 acc = bias4(s.b3 + 16 * t + 4 * g);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(s.w3 + (16 * t + r) * kHid + 16 * k + 4 * g), hb[k], acc,
+                                                            0, 0, 0);
+        ha[t] = pack_relu(acc);
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        f32x4 acc = bias4(s.b4 + 16 * t + 4 * g);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(s.w4 + (16 * t + r) * kHid + 16 * k + 4 * g), ha[k], acc,
+                                                            0, 0, 0);
+        y[t] = acc;
+    }
+}
+
+// Per-wave staging of 64 encoded samples (bf16 rows of 32 features, padded
+// to 36 so the B-operand reads of 16 rows spread over the LDS banks).
+constexpr int kStageRow = kEnc + 4;
+struct WaveStage {
+    unsigned short x[64 * kStageRow];
+};
+
+// Lane l stores its sample's 32 features (bf16) as row l of the stage.
+__device__ inline void stage_row(WaveStage& st, const float f[kEnc]) {
+    uint2* row = reinterpret_cast<uint2*>(st.x + __lane_id() * kStageRow);
+#pragma unroll
+    for (int i = 0; i < kEnc / 4; ++i)
+        row[i] = make_uint2((uint32_t)f2bf(f[4 * i]) | ((uint32_t)f2bf(f[4 * i + 1]) << 16),
+                            (uint32_t)f2bf(f[4 * i + 2]) | ((uint32_t)f2bf(f[4 * i + 3]) << 16));
+}
+
+// Writes the 40 raw outputs of one sample from the column-form result.
+__device__ inline void store_outputs(float* dst, const f32x4 y[3]) {
+    const int g = __lane_id() >> 4;
+    *reinterpret_cast<float4*>(dst + 4 * g) = make_float4(y[0][0], y[0][1], y[0][2], y[0][3]);
+    *reinterpret_cast<float4*>(dst + 16 + 4 * g) = make_float4(y[1][0], y[1][1], y[1][2], y[1][3]);
+    if (g < 2) *reinterpret_cast<float4*>(dst + 32 + 4 * g) = make_float4(y[2][0], y[2][1], y[2][2], y[2][3]);
+}
+
+// Forward of the 64 samples staged by this wave: for sample group q (16
+// samples 16 q .. 16 q + 15) calls out(q, y) with lane (r, g) holding outputs
+// 16 t + 4 g + i of sample 16 q + r.
+template <typename Out>
+__device__ inline void mlp_stage_fwd(const SharedWeights& s, const WaveStage& st, Out out) {
+    const int l = __lane_id(), r = l & 15, g = l >> 4;
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        const unsigned short* row = st.x + (16 * q + r) * kStageRow;
+        const bf16x4 x[2] = {ld4(row + 4 * g), ld4(row + 16 + 4 * g)};
+        f32x4 y[3];
+        mlp_cols_fwd(s, x, y);
+        out(q, y);
+    }
 }
 
 // Mixture parameters of one raw output row.

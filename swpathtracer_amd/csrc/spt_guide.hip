@@ -12,41 +12,37 @@ using namespace guide;
 
 namespace {
 
-// Rows [base, base + 16) of the queries: (p, n, wo) as 9 floats with a stride.
+// Raw outputs for queries (p, n, wo) as 9 floats with a stride: each wave
+// encodes 64 queries (one per lane) into its stage, then runs the
+// register-resident MLP on 4 groups of 16.
 __global__ __launch_bounds__(kBlock) void k_guide_eval(const float* __restrict__ theta,
                                                        const unsigned short* __restrict__ wbf,
                                                        const float* __restrict__ q, uint32_t stride, uint32_t n,
                                                        float* __restrict__ out, GuideBox box) {
     extern __shared__ float dyn[];
     SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
-    WaveTiles* tiles = reinterpret_cast<WaveTiles*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
+    WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
     load_weights(sw, wbf, theta);
     __syncthreads();
-    const int wave = threadIdx.x >> 6, l = __lane_id();
-    WaveTiles& t = tiles[wave];
-    const uint32_t ntiles = (n + 15) / 16;
-    for (uint32_t base = blockIdx.x * (kBlock / 64); base < ntiles; base += gridDim.x * (kBlock / 64)) {
-        const uint32_t tile = base + wave;
-        if (tile < ntiles && l < 16) {
-            const uint32_t row = tile * 16 + l;
-            float f[kEnc];
-            if (row < n) {
-                const float* r = q + (size_t)row * stride;
-                encode(V3{r[0], r[1], r[2]}, V3{r[3], r[4], r[5]}, V3{r[6], r[7], r[8]}, box.bmin, box.bext, f);
-            } else {
-                for (int i = 0; i < kEnc; ++i) f[i] = 0.0f;
-            }
-            for (int i = 0; i < kEnc; ++i) t.x[l * kEnc + i] = f2bf(f[i]);
+    const int wave = threadIdx.x >> 6, r = __lane_id() & 15;
+    WaveStage& st = stages[wave];
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t row = base + threadIdx.x;
+        float f[kEnc];
+        if (row < n) {
+            const float* p = q + (size_t)row * stride;
+            encode(V3{p[0], p[1], p[2]}, V3{p[3], p[4], p[5]}, V3{p[6], p[7], p[8]}, box.bmin, box.bext, f);
+        } else {
+            for (int i = 0; i < kEnc; ++i) f[i] = 0.0f;
         }
+        stage_row(st, f);
         __syncthreads();
-        if (tile < ntiles) mlp_tile_fwd(sw, t);
-        __syncthreads();
-        if (tile < ntiles) {
-            for (int idx = l; idx < 16 * kOut; idx += 64) {
-                const uint32_t row = tile * 16 + idx / kOut;
-                if (row < n) out[(size_t)row * kOut + idx % kOut] = t.y[(idx / kOut) * kOutP + idx % kOut];
-            }
-        }
+        const uint32_t wbase = base + (uint32_t)wave * 64;
+        if (wbase < n)
+            mlp_stage_fwd(sw, st, [&](int g16, const f32x4 y[3]) {
+                const uint32_t smp = wbase + 16 * g16 + r;
+                if (smp < n) store_outputs(out + (size_t)smp * kOut, y);
+            });
         __syncthreads();
     }
 }
@@ -171,7 +167,7 @@ __global__ void k_guide_tobf16(const float* __restrict__ theta, unsigned short* 
     if (i < (uint32_t)kParams) wbf[i] = f2bf(theta[i]);
 }
 
-size_t eval_lds_bytes() { return sizeof(SharedWeights) + sizeof(WaveTiles) * (kBlock / 64); }
+size_t eval_lds_bytes() { return sizeof(SharedWeights) + sizeof(WaveStage) * (kBlock / 64); }
 size_t train_lds_bytes() { return sizeof(SharedWeights) + sizeof(float) * kParams + sizeof(WaveTiles) * (kBlock / 64); }
 
 // The training and evaluation kernels use more than the default 64 KiB of
@@ -207,8 +203,7 @@ spt_status guide_eval_launch(spt_ctx* ctx, const float* d_q, uint32_t stride, ui
     GuideState& gs = ctx->guide;
     spt_status ls = allow_lds(ctx);
     if (ls != SPT_OK) return ls;
-    const uint32_t tiles = (n + 15) / 16;
-    const uint32_t grid = std::min<uint32_t>((tiles + 3) / 4, (uint32_t)ctx->num_cus * 4);
+    const uint32_t grid = std::min<uint32_t>((n + kBlock - 1) / kBlock, (uint32_t)ctx->num_cus * 4);
     hipLaunchKernelGGL(k_guide_eval, dim3(grid), dim3(kBlock), eval_lds_bytes(), ctx->stream, gs.d_theta, gs.d_wbf,
                        d_q, stride, n, d_out, guide_box(ctx));
     return check_hip(ctx, hipGetLastError(), "guide eval launch");

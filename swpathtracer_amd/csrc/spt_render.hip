@@ -347,7 +347,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RenderArgs a, Work w, const 
 }
 
 // Guide inference for the live hits of buffer A whose material is guidable,
-// 16 rows per wave in shading order; raw outputs go to gout[slot].
+// in shading order: each wave encodes 64 hits (one per lane) and runs the
+// register-resident MLP; raw outputs go to gout[slot].
 __global__ __launch_bounds__(kBlock) void k_guide_paths(RenderArgs a, PathBuf A, Work w,
                                                         const uint32_t* __restrict__ cntA,
                                                         const float* __restrict__ theta,
@@ -355,51 +356,46 @@ __global__ __launch_bounds__(kBlock) void k_guide_paths(RenderArgs a, PathBuf A,
     using namespace guide;
     extern __shared__ float dyn[];
     SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
-    WaveTiles* tiles = reinterpret_cast<WaveTiles*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
+    WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
     const uint32_t n = *cntA;
     if (n == 0) return;
     load_weights(sw, wbf, theta);
     __syncthreads();
-    const int wave = threadIdx.x >> 6, l = __lane_id();
-    WaveTiles& t = tiles[wave];
-    const uint32_t ntiles = (n + 15) / 16;
-    for (uint32_t base = blockIdx.x * (kBlock / 64); base < ntiles; base += gridDim.x * (kBlock / 64)) {
-        const uint32_t tile = base + wave;
+    const int wave = threadIdx.x >> 6, l = __lane_id(), r = l & 15;
+    WaveStage& st = stages[wave];
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t j = base + threadIdx.x;
         bool mine = false;
         uint32_t slot = 0;
-        if (tile < ntiles && l < 16) {
-            const uint32_t j = tile * 16 + l;
-            float f[kEnc];
-            if (j < n) {
-                slot = a.keys ? w.perm[j] : j;
-                const float4 hv = w.hit[slot];
-                const uint32_t idx = __float_as_uint(hv.y);
-                mine = (idx != kNoHit) && a.guidable[w.skey[slot]];
-                if (mine) {
-                    const float4 o4 = A.o[slot], d4 = A.d[slot];
-                    const V3 d{d4.x, d4.y, d4.z};
-                    Hit h;
-                    h.t = hv.x;
-                    h.idx = idx;
-                    h.u = hv.z;
-                    h.v = hv.w;
-                    const V3 pos = V3{o4.x, o4.y, o4.z} + d * h.t;
-                    encode(pos, shading_normal(a.sc, h), -d, box.bmin, box.bext, f);
-                }
+        float f[kEnc];
+        if (j < n) {
+            slot = a.keys ? w.perm[j] : j;
+            const float4 hv = w.hit[slot];
+            const uint32_t idx = __float_as_uint(hv.y);
+            mine = (idx != kNoHit) && a.guidable[w.skey[slot]];
+            if (mine) {
+                const float4 o4 = A.o[slot], d4 = A.d[slot];
+                const V3 d{d4.x, d4.y, d4.z};
+                Hit h;
+                h.t = hv.x;
+                h.idx = idx;
+                h.u = hv.z;
+                h.v = hv.w;
+                const V3 pos = V3{o4.x, o4.y, o4.z} + d * h.t;
+                encode(pos, shading_normal(a.sc, h), -d, box.bmin, box.bext, f);
             }
-            if (!mine)
-                for (int i = 0; i < kEnc; ++i) f[i] = 0.0f;
-            for (int i = 0; i < kEnc; ++i) t.x[l * kEnc + i] = f2bf(f[i]);
         }
+        if (!mine)
+            for (int i = 0; i < kEnc; ++i) f[i] = 0.0f;
         const uint64_t any = __ballot(mine);
+        stage_row(st, f);
         __syncthreads();
-        if (tile < ntiles && any) mlp_tile_fwd(sw, t);
-        __syncthreads();
-        if (tile < ntiles && any) {
-            // lane r (< 16) writes its row's 40 outputs
-            if (mine)
-                for (int c = 0; c < kOut; ++c) a.gout[(size_t)slot * kOut + c] = t.y[l * kOutP + c];
-        }
+        if (any)
+            mlp_stage_fwd(sw, st, [&](int g16, const f32x4 y[3]) {
+                const int src = 16 * g16 + r;
+                const uint32_t s = __shfl(slot, src);
+                if ((any >> src) & 1ull) store_outputs(a.gout + (size_t)s * kOut, y);
+            });
         __syncthreads();
     }
 }
@@ -553,7 +549,7 @@ __global__ void k_flip(uint32_t* cntA, uint32_t* fetch) {
 uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 size_t guide_paths_lds_bytes() {
-    return sizeof(guide::SharedWeights) + sizeof(guide::WaveTiles) * (kBlock / 64);
+    return sizeof(guide::SharedWeights) + sizeof(guide::WaveStage) * (kBlock / 64);
 }
 
 }  // namespace
