@@ -103,11 +103,12 @@ def _rmse(img, ref):
 
 def guide_experiment(args):
     """Time to equal RMSE on the slit-lit scene: the plain tracer renders
-    1024 spp (timed); the guided tracer renders progressive passes, training
-    the guide online after each pass, until its running image reaches the
-    plain image's RMSE against an FP64-film reference of oracle_spp samples
-    (independent seed). Only render and training time is counted; RMSE
-    evaluation between passes is not."""
+    1024 spp (timed); the guided tracer renders warm-up passes that only
+    train the guide, then counted passes (training continues after each)
+    until its running image reaches the plain image's RMSE against an
+    FP64-film reference of oracle_spp samples (independent seed). All render
+    and training time is counted, warm-up included; RMSE evaluation between
+    passes is not."""
     from swpathtracer_amd import api, scenes
     import swpathtracer_amd._lib as L
 
@@ -156,40 +157,59 @@ def guide_experiment(args):
     r_plain = _rmse(img_plain, ref)
     log(f"plain 1024 spp: {t_plain * 1e3:.1f} ms, RMSE {r_plain:.5f}")
 
-    # guided: progressive passes with online training
-    dev.guide_init(seed=1, lr=args.guide_lr, batch=262144)
-    film.zero()
-    total_spp = 4096
-    done, t_guided, t_train, hit = 0, 0.0, 0.0, None
+    # guided: warm-up passes (1 spp each, guide trained after every pass,
+    # images discarded but their time counted), then passes of 4 spp that
+    # keep training; the running image of the post-warm-up passes is compared
+    # with the reference after each pass (comparison time not counted)
+    dev.guide_init(seed=1, lr=args.guide_lr, batch=262144, steps_per_pass=args.guide_steps)
+    t_guided, t_train, hit = 0.0, 0.0, None
     curve = []
-    # each pass is a complete jittered estimate of c spp (c square) with its own seed
-    chunk_sched = [1] * 16 + [4] * 8
     k = 0
-    while done < total_spp:
-        c = chunk_sched[k] if k < len(chunk_sched) else 16
+
+    def guided_pass(c, tgt):
+        nonlocal k, t_guided, t_train
         k += 1
         p = integ.params()
-        p.spp, p.seed = c, 0x600D0000 + k
+        p.spp, p.seed = c, 0x600D0000 + k  # a complete jittered estimate of c spp
         p.flags = L.SPT_RENDER_FILM32 | L.SPT_RENDER_GUIDED | L.SPT_RENDER_GUIDE_TRAIN
         dev.sync()
         t = time.perf_counter()
-        dev.render(cam, p, film.ptr, None, None)
+        dev.render(cam, p, tgt.ptr, None, None)
         t1 = time.perf_counter()
-        dev.guide_train_collected()
+        dev.guide_train_collected(wait=False)
         dev.sync()
         t2 = time.perf_counter()
         t_guided += t2 - t
         t_train += t2 - t1
+
+    warm = dev.alloc(npix * 3 * 4)
+    warm.zero()
+    for _ in range(args.guide_warmup):
+        guided_pass(1, warm)
+    t_warm = t_guided
+    # counted passes run back to back; after each, the running film is copied
+    # on the device and its RMSE is evaluated only after the run, so the
+    # measurement leaves the GPU no idle gaps between passes
+    film.zero()
+    c = args.guide_pass_spp
+    snaps, marks = [], []
+    done = 0
+    while done < 4096 and t_guided < 2.0 * t_plain:
+        guided_pass(c, film)
         done += c
-        img = film.download(np.float32, (H, W, 3)) / done
-        r = _rmse(img, ref)
-        curve.append((done, round(t_guided * 1e3, 2), round(r, 6)))
+        snap = dev.alloc(npix * 3 * 4)
+        dev._check(dev.lib.spt_memcpy_d2d(dev.h, snap.ptr, film.ptr, npix * 12))
+        snaps.append(snap)
+        marks.append((done, t_guided))
+    dev.sync()
+    host = np.empty((H, W, 3), np.float32)
+    for snap, (spp_k, t_k) in zip(snaps, marks):
+        snap.download(np.float32, (H, W, 3), out=host)
+        r = _rmse(host / spp_k, ref)
+        curve.append((spp_k, round(t_k * 1e3, 2), round(r, 6)))
         if hit is None and r <= r_plain:
-            hit = (done, t_guided, r)
-            if not args.guide_full:
-                break
-        if t_guided > 3 * t_plain and hit is None and not args.guide_full:
-            break
+            hit = (spp_k, t_k, r)
+        snap.free()
     st = dev.guide_stats()
     flops = st["trained_records"] * 76.8e3
     mfma = flops / max(st["train_kernel_ms"] * 1e-3, 1e-12) / 2.5e15
@@ -200,6 +220,7 @@ def guide_experiment(args):
            "guided_ms_to_equal_rmse": round(hit[1] * 1e3, 2) if hit else None,
            "time_ratio": round(hit[1] / t_plain, 4) if hit else None,
            "guided_train_ms": round(t_train * 1e3, 2), "train_steps": st["steps"],
+           "warmup_passes": args.guide_warmup, "warmup_ms": round(t_warm * 1e3, 2),
            "train_kernel_ms": round(st["train_kernel_ms"], 3), "trained_records": st["trained_records"],
            "train_mfma_util": round(mfma, 6), "final_loss": st["last_loss"],
            "curve": curve[:: max(1, len(curve) // 24)]}
@@ -263,6 +284,9 @@ def main():
     ap.add_argument("--guide-res", type=int, nargs=2, default=(800, 600))
     ap.add_argument("--oracle-spp", type=int, default=16384)
     ap.add_argument("--guide-lr", type=float, default=5e-3)
+    ap.add_argument("--guide-warmup", type=int, default=32, help="1-spp training passes before the counted image")
+    ap.add_argument("--guide-pass-spp", type=int, default=16, help="spp of each counted guided pass (a square)")
+    ap.add_argument("--guide-steps", type=int, default=4, help="Adam steps per training pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()

@@ -411,7 +411,10 @@ typedef struct spt_guide_config {
     float learning_rate;   /* Adam step (default 5e-3) */
     float loss_scale;      /* initial loss scale (default 65536; halved on overflow) */
     float bsdf_fraction;   /* one-sample mixture weight of BSDF sampling (default 0.5) */
-    uint32_t batch;        /* training records per step / record capacity (default 262144) */
+    uint32_t batch;        /* training records per step (default 262144) */
+    uint32_t steps_per_pass; /* a training render records up to batch x steps_per_pass
+                                records; spt_guide_train_collected takes one Adam step
+                                per batch of them (default 1) */
 } spt_guide_config;
 
 typedef struct spt_guide_stats {
@@ -441,13 +444,17 @@ spt_status spt_guide_get_params(spt_ctx* ctx, float* theta);
  * normal, outgoing direction; world space). Needs an uploaded scene (its box
  * normalises positions). */
 spt_status spt_guide_eval(spt_ctx* ctx, const float* d_queries, uint64_t n, float* d_out);
-/* One Adam step on n records (n x SPT_GUIDE_RECORD_FLOATS: position 0-2,
+/* One Adam step on n records (loss may be NULL: no wait) (n x SPT_GUIDE_RECORD_FLOATS: position 0-2,
  * normal 3-5, wo 6-8, wi 9-11, weight 12). loss may be NULL. */
 spt_status spt_guide_train(spt_ctx* ctx, const float* d_records, uint32_t n, float* loss);
 /* The (unscaled) gradient the step would apply, without applying it (host
  * grad: SPT_GUIDE_PARAMS floats). */
 spt_status spt_guide_gradient(spt_ctx* ctx, const float* d_records, uint32_t n, float* grad, float* loss);
-/* One Adam step on the records collected by the last SPT_RENDER_GUIDE_TRAIN render. */
+/* Adam steps on the records collected by the last SPT_RENDER_GUIDE_TRAIN
+ * render: one per `batch` records (up to steps_per_pass). Steps are queued on
+ * the context stream; with loss != NULL the call waits and returns the loss of
+ * the last step, with NULL it returns at once. Skips and loss-scale changes
+ * are decided on the device (spt_guide_get_stats reports them). */
 spt_status spt_guide_train_collected(spt_ctx* ctx, float* loss);
 spt_status spt_guide_get_stats(spt_ctx* ctx, spt_guide_stats* out);
 /* Device pointer and count of the collected records. */

@@ -108,8 +108,10 @@ def mixture_pdf(y, d):
 
 
 def sample_vmf(mu, kappa, u1, u2):
-    """Jakob 2012: w = 1 + log(u1 + (1 - u1) e^{-2 kappa}) / kappa."""
-    wz = 1.0 + np.log(u1 + (1.0 - u1) * np.exp(-2.0 * kappa)) / kappa
+    """Jakob 2012: w = 1 + log(u1 + (1 - u1) e^{-2 kappa}) / kappa, clamped to
+    its limit -1 where the argument underflows (u1 = 0, large kappa)."""
+    with np.errstate(divide="ignore"):
+        wz = np.maximum(1.0 + np.log(u1 + (1.0 - u1) * np.exp(-2.0 * kappa)) / kappa, -1.0)
     st = np.sqrt(np.maximum(0.0, 1.0 - wz * wz))
     ph = 2 * np.pi * u2
     local = np.stack([st * np.cos(ph), st * np.sin(ph), wz], -1)

@@ -212,8 +212,12 @@ class DeviceArray:
         self.dev._check(self.dev.lib.spt_memcpy_h2d(self.dev.h, self.ptr, a.ctypes.data, a.nbytes))
         self.dev.sync()
 
-    def download(self, dtype, shape) -> np.ndarray:
-        out = np.empty(shape, dtype)
+    def download(self, dtype, shape, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """Copies to a new host array, or into `out` (reusing one host buffer
+        avoids mapping fresh pages for every copy in loops)."""
+        if out is None:
+            out = np.empty(shape, dtype)
+        assert out.dtype == np.dtype(dtype) and out.flags.c_contiguous
         assert out.nbytes <= self.nbytes
         self.dev._check(self.dev.lib.spt_memcpy_d2h(self.dev.h, out.ctypes.data, self.ptr, out.nbytes))
         return out
@@ -358,9 +362,9 @@ class Device:
 
     # learned guide -----------------------------------------------------
     def guide_init(self, seed: int = 1, lr: float = 5e-3, loss_scale: float = 65536.0, bsdf_fraction: float = 0.5,
-                   batch: int = 262144):
+                   batch: int = 262144, steps_per_pass: int = 1):
         cfg = L.GuideConfig(seed=seed, learning_rate=lr, loss_scale=loss_scale, bsdf_fraction=bsdf_fraction,
-                            batch=batch)
+                            batch=batch, steps_per_pass=steps_per_pass)
         self._check(self.lib.spt_guide_init(self.h, C.byref(cfg)))
 
     def guide_set_params(self, theta: np.ndarray):
@@ -398,7 +402,11 @@ class Device:
         self._check(self.lib.spt_guide_train(self.h, dr.ptr, n, C.byref(loss)))
         return loss.value
 
-    def guide_train_collected(self) -> float:
+    def guide_train_collected(self, wait: bool = True) -> Optional[float]:
+        """Queues the training steps; with wait, returns the last step's loss."""
+        if not wait:
+            self._check(self.lib.spt_guide_train_collected(self.h, None))
+            return None
         loss = C.c_float()
         self._check(self.lib.spt_guide_train_collected(self.h, C.byref(loss)))
         return loss.value

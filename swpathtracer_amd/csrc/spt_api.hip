@@ -21,6 +21,23 @@ spt_status check_hip(spt_ctx* ctx, hipError_t e, const char* what) {
     return set_error(ctx, e == hipErrorOutOfMemory ? SPT_ERR_NOMEM : SPT_ERR_DEVICE, m);
 }
 
+spt_status stream_wait(spt_ctx* ctx, const char* what) {
+    while (true) {
+        const hipError_t e = hipStreamQuery(ctx->stream);
+        if (e == hipSuccess) return SPT_OK;
+        if (e != hipErrorNotReady) return check_hip(ctx, e, what);
+        __builtin_ia32_pause();
+    }
+}
+
+spt_status read_small(spt_ctx* ctx, void* dst, const void* d_src, size_t bytes, const char* what) {
+    if (bytes > 4096 || !ctx->h_pin) return set_error(ctx, SPT_ERR_INVALID, "read_small: more than 4 KiB");
+    spt_status s = check_hip(ctx, hipMemcpyAsync(ctx->h_pin, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream), what);
+    if (s == SPT_OK) s = stream_wait(ctx, what);
+    if (s == SPT_OK) std::memcpy(dst, ctx->h_pin, bytes);
+    return s;
+}
+
 spt_status ensure_spill(spt_ctx* ctx, size_t words) {
     if (words <= ctx->spill_words) return SPT_OK;
     if (ctx->d_spill) {
@@ -163,7 +180,9 @@ spt_status spt_create(int device, spt_ctx** out) {
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         ctx->wall_clock_khz = khz;
-    if (hipMalloc(&ctx->d_flags, 64) != hipSuccess || hipMemset(ctx->d_flags, 0, 64) != hipSuccess) {
+    if (hipHostMalloc(&ctx->h_pin, 4096, hipHostMallocDefault) != hipSuccess) ctx->h_pin = nullptr;
+    if (!ctx->h_pin || hipMalloc(&ctx->d_flags, 64) != hipSuccess || hipMemset(ctx->d_flags, 0, 64) != hipSuccess) {
+        if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return SPT_ERR_DEVICE;
@@ -190,8 +209,8 @@ void spt_destroy(spt_ctx* ctx) {
     if (ctx->guide.d_theta) (void)hipFree(ctx->guide.d_theta);
     if (ctx->guide.d_records) (void)hipFree(ctx->guide.d_records);
     if (ctx->guide.d_gout) (void)hipFree(ctx->guide.d_gout);
-    for (hipEvent_t e : ctx->guide.ev)
-        if (e) (void)hipEventDestroy(e);
+    ctx->guide.destroy_events();
+    if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
     if (ctx->d_guidable) (void)hipFree(ctx->d_guidable);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -216,10 +235,8 @@ const char* spt_last_error(const spt_ctx* ctx) { return ctx ? ctx->err.c_str() :
 
 spt_status spt_sync(spt_ctx* ctx) {
     if (!ctx) return SPT_ERR_INVALID;
-    spt_status s = check_hip(ctx, hipStreamSynchronize(ctx->stream), "stream sync");
-    if (s != SPT_OK) return s;
     uint32_t flags = 0;
-    s = check_hip(ctx, hipMemcpy(&flags, ctx->d_flags, sizeof(flags), hipMemcpyDeviceToHost), "flag read");
+    spt_status s = read_small(ctx, &flags, ctx->d_flags, sizeof(flags), "stream sync");
     if (s != SPT_OK) return s;
     if (flags & 1u) {
         (void)hipMemset(ctx->d_flags, 0, 64);
@@ -250,7 +267,7 @@ spt_status spt_memcpy_d2h(spt_ctx* ctx, void* dst, const void* d_src, size_t byt
     if (!ctx || (!dst && bytes) || (!d_src && bytes)) return SPT_ERR_INVALID;
     spt_status s = check_hip(ctx, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     if (s != SPT_OK) return s;
-    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "d2h sync");
+    return stream_wait(ctx, "d2h sync");
 }
 
 spt_status spt_memcpy_d2d(spt_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {

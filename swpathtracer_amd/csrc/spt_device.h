@@ -247,7 +247,10 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                     L.sp = 0;
                     if (FETCH) L.nodes = L.tris = 0;
                     float tn, tf;
-                    if (!slab(L.o, L.inv, R.mn, R.mx, tn, tf))
+                    // a ray with a non-finite component cannot hit anything (every
+                    // t comparison fails) but would pass every box test: a miss
+                    const float sum = L.o.x + L.o.y + L.o.z + L.d.x + L.d.y + L.d.z;
+                    if (!slab(L.o, L.inv, R.mn, R.mx, tn, tf) || !(sum - sum == 0.0f))
                         L.cur = kDone;
                     else
                         L.cur = R.leaf ? kLeafBit : R.node0;
@@ -422,30 +425,75 @@ struct GuideBox {
     float bmin[3], bext[3];
 };
 
+// Device-resident training state of the guide (64-float area after the
+// gradient): the host never waits for a step; it reads this block when it
+// asks for statistics or a loss.
+struct GuideCtl {
+    float wsum;       // sum of the current step's record weights
+    float loss;       // loss of the current step
+    uint32_t bad;     // a gradient entry was not finite
+    float scale;      // loss scale (halved on overflow)
+    uint32_t steps;   // successful steps (Adam step number - 1)
+    uint32_t skipped; // skipped steps
+    float last_loss;  // loss of the last successful step
+    uint32_t pad;
+    unsigned long long trained;  // records consumed by successful steps
+};
+
 // Learned guide state of a context (spt_guide.hip).
 struct GuideState {
     bool ready = false;
-    float* d_theta = nullptr;  // fp32 master parameters, then m, v, grad, acc (64 floats)
+    float* d_theta = nullptr;  // fp32 master parameters, then m, v, grad, control block (64 floats)
     float* d_m = nullptr;
     float* d_v = nullptr;
     float* d_grad = nullptr;
     float* d_acc = nullptr;
     unsigned short* d_wbf = nullptr;  // bf16 copy of the parameters
     float* d_records = nullptr;
-    uint32_t records_cap = 0;
+    uint32_t records_cap = 0;    // batch x steps_per_pass
+    uint32_t records_alloc = 0;  // allocated record slots
     uint64_t collected = 0;
-    float lr = 5e-3f, loss_scale = 65536.0f, bsdf_fraction = 0.5f;
+    float lr = 5e-3f, loss_scale = 65536.0f, bsdf_fraction = 0.5f;  // loss_scale: the configured start
     uint32_t batch = 262144;
-    uint64_t step = 0, skipped = 0;
-    float last_loss = 0.0f;
     double train_ms = 0.0;
     float* d_gout = nullptr;  // per-path raw outputs during a guided render
     size_t gout_cap = 0;
     uint32_t rec_stride = 1;      // 1 in rec_stride candidate vertices is recorded
+    double cand_per_path = 0.0;   // candidate vertices per path of the last training render
     uint64_t candidates = 0;      // candidate vertices of the last training render
-    uint64_t trained = 0;         // records consumed by successful steps
-    double kernel_ms = 0.0;       // device time of k_guide_train
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    double kernel_ms = 0.0;       // device time of k_guide_train (settled steps)
+    std::vector<hipEvent_t> ev_free, ev_pending;  // event pairs around training kernels
+
+    GuideCtl* ctl() const { return reinterpret_cast<GuideCtl*>(d_acc); }
+    // an event pair for one training kernel (false: none available)
+    bool event_pair(hipEvent_t& a, hipEvent_t& b) {
+        for (int k = 0; k < 2; ++k) {
+            if (ev_free.empty()) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return false;
+                ev_free.push_back(e);
+            }
+            (k == 0 ? a : b) = ev_free.back();
+            ev_free.pop_back();
+            ev_pending.push_back(k == 0 ? a : b);
+        }
+        return true;
+    }
+    // adds the device time of completed pairs (call after the stream is idle)
+    void settle_events() {
+        for (size_t i = 0; i + 1 < ev_pending.size(); i += 2) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev_pending[i], ev_pending[i + 1]) == hipSuccess) kernel_ms += ms;
+        }
+        ev_free.insert(ev_free.end(), ev_pending.begin(), ev_pending.end());
+        ev_pending.clear();
+    }
+    void destroy_events() {
+        for (hipEvent_t e : ev_free) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_pending) (void)hipEventDestroy(e);
+        ev_free.clear();
+        ev_pending.clear();
+    }
 };
 
 }  // namespace spt
@@ -486,6 +534,7 @@ struct spt_ctx {
     spt::GuideState guide;
     spt_tuning tuning{};  // zero fields: defaults
     int wall_clock_khz = 100000;
+    void* h_pin = nullptr;  // 4 KiB of pinned host memory for small read-backs
 
     spt::DevScene scene() const {
         spt::DevScene s;
@@ -509,6 +558,11 @@ struct spt_ctx {
 namespace spt {
 spt_status set_error(spt_ctx* ctx, spt_status code, const std::string& msg);
 spt_status check_hip(spt_ctx* ctx, hipError_t e, const char* what);
+// Waits for the context stream by polling it (no blocking wait on an
+// interrupt: those have been seen to wake up hundreds of ms late).
+spt_status stream_wait(spt_ctx* ctx, const char* what);
+// Copies up to 4 KiB from the device through pinned memory and waits.
+spt_status read_small(spt_ctx* ctx, void* dst, const void* d_src, size_t bytes, const char* what);
 // grows ctx->d_spill to hold `words` entries; returns false on failure
 spt_status ensure_spill(spt_ctx* ctx, size_t words);
 // launch helpers implemented by the kernel translation units

@@ -322,7 +322,10 @@ __device__ inline V3 mixture_sample(const Lobe L[kLobes], float u0, float u1, fl
         }
     }
     const float kap = L[k].kappa;
-    const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kap)) / kap;
+    // for kappa above ~44, e^{-2 kappa} underflows and u1 = 0 gives log(0):
+    // the exact limit is w = -1, so clamp (a NaN direction would make the
+    // next ray visit every node of the tree)
+    const float w = fmaxf(1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kap)) / kap, -1.0f);
     const float st = sqrtf(fmaxf(0.0f, 1.0f - w * w));
     const float ph = 2.0f * kPi * u2;
     V3 t, b;

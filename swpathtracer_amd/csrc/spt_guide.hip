@@ -57,14 +57,14 @@ __global__ __launch_bounds__(kBlock) void k_guide_wsum(const float* __restrict__
 }
 
 // One training step's gradient: forward, per-row KL-loss gradient (weights
-// normalised by the batch mean, averaged over the batch, times loss_scale),
-// backward; weight gradients summed in LDS then added to grad (fp32).
-// acc[0] = sum of weights (input), acc[1] += loss.
+// normalised by the batch mean, averaged over the batch, times the loss
+// scale), backward; weight gradients summed in LDS then added to grad (fp32).
+// Reads ctl->wsum and ctl->scale, adds the step's loss to ctl->loss.
 __global__ __launch_bounds__(kBlock) void k_guide_train(const float* __restrict__ theta,
                                                         const unsigned short* __restrict__ wbf,
                                                         const float* __restrict__ rec, uint32_t n,
-                                                        float* __restrict__ grad, float* __restrict__ acc,
-                                                        float loss_scale, GuideBox box) {
+                                                        float* __restrict__ grad, GuideCtl* __restrict__ ctl,
+                                                        GuideBox box) {
     extern __shared__ float dyn[];
     SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
     float* dW = reinterpret_cast<float*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
@@ -74,8 +74,8 @@ __global__ __launch_bounds__(kBlock) void k_guide_train(const float* __restrict_
     __syncthreads();
     const int wave = threadIdx.x >> 6, l = __lane_id();
     WaveTiles& t = tiles[wave];
-    const float wmean = fmaxf(acc[0] / (float)n, 1e-30f);
-    const float scale = loss_scale / (float)n;
+    const float wmean = fmaxf(ctl->wsum / (float)n, 1e-30f);
+    const float scale = ctl->scale / (float)n;
     float loss = 0.0f;
     const uint32_t ntiles = (n + 15) / 16;
     for (uint32_t base = blockIdx.x * (kBlock / 64); base < ntiles; base += gridDim.x * (kBlock / 64)) {
@@ -131,27 +131,40 @@ __global__ __launch_bounds__(kBlock) void k_guide_train(const float* __restrict_
         __syncthreads();
     }
     for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
-    if (l == 0) atomicAdd(&acc[1], loss / (float)n);
+    if (l == 0) atomicAdd(&ctl->loss, loss / (float)n);
     __syncthreads();
     for (int i = threadIdx.x; i < kParams; i += blockDim.x)
         if (dW[i] != 0.0f) atomicAdd(&grad[i], dW[i]);
 }
 
-// flag[0] = 1 when any gradient entry is not finite (loss-scale overflow)
-__global__ __launch_bounds__(kBlock) void k_guide_check(const float* __restrict__ grad, uint32_t* flag) {
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < (uint32_t)kParams; i += gridDim.x * kBlock)
-        if (!isfinite(grad[i])) atomicOr(flag, 1u);
+// Start of a step: clears the per-step fields of the control block.
+__global__ void k_guide_begin(GuideCtl* ctl) {
+    if (threadIdx.x == 0) {
+        ctl->wsum = 0.0f;
+        ctl->loss = 0.0f;
+        ctl->bad = 0u;
+    }
 }
 
-// FP32 Adam on the unscaled gradient; refreshes the bf16 copy; clears grad.
+// ctl->bad = 1 when any gradient entry is not finite (loss-scale overflow)
+__global__ __launch_bounds__(kBlock) void k_guide_check(const float* __restrict__ grad, GuideCtl* ctl) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < (uint32_t)kParams; i += gridDim.x * kBlock)
+        if (!isfinite(grad[i])) atomicOr(&ctl->bad, 1u);
+}
+
+// FP32 Adam on the unscaled gradient, skipped when the gradient overflowed or
+// the batch carried no weight; refreshes the bf16 copy and clears grad. The
+// step number (for the bias corrections) is the count of successful steps + 1.
 __global__ __launch_bounds__(kBlock) void k_guide_adam(float* __restrict__ theta, float* __restrict__ m,
                                                        float* __restrict__ v, float* __restrict__ grad,
-                                                       unsigned short* __restrict__ wbf, const uint32_t* flag,
-                                                       float lr, float inv_scale, float bc1, float bc2) {
+                                                       unsigned short* __restrict__ wbf, const GuideCtl* ctl,
+                                                       float lr) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= (uint32_t)kParams) return;
-    if (*flag == 0u) {
-        const float g = grad[i] * inv_scale;
+    if (ctl->bad == 0u && ctl->wsum > 0.0f) {
+        const float t = (float)(ctl->steps + 1u);
+        const float bc1 = 1.0f - powf(0.9f, t), bc2 = 1.0f - powf(0.99f, t);
+        const float g = grad[i] * (1.0f / ctl->scale);
         const float mi = 0.9f * m[i] + 0.1f * g;
         const float vi = 0.99f * v[i] + 0.01f * g * g;
         m[i] = mi;
@@ -160,6 +173,19 @@ __global__ __launch_bounds__(kBlock) void k_guide_adam(float* __restrict__ theta
     }
     wbf[i] = f2bf(theta[i]);
     grad[i] = 0.0f;
+}
+
+// End of a step: counters, last loss, and the loss scale halved on overflow.
+__global__ void k_guide_post(GuideCtl* ctl, uint32_t n) {
+    if (threadIdx.x != 0) return;
+    if (ctl->bad == 0u && ctl->wsum > 0.0f) {
+        ctl->steps += 1u;
+        ctl->trained += n;
+        ctl->last_loss = ctl->loss;
+    } else {
+        ctl->skipped += 1u;
+        if (ctl->bad) ctl->scale *= 0.5f;
+    }
 }
 
 __global__ void k_guide_tobf16(const float* __restrict__ theta, unsigned short* __restrict__ wbf) {
@@ -215,46 +241,26 @@ spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, floa
     if (ls != SPT_OK) return ls;
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = ctx->stream;
-    spt_status s = check_hip(ctx, hipMemsetAsync(gs.d_acc, 0, 64, st), "guide acc");
-    if (s != SPT_OK) return s;
-    hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, gs.d_acc);
-    const size_t lds = train_lds_bytes();
-    if (gs.ev[0]) (void)hipEventRecord(gs.ev[0], st);
-    hipLaunchKernelGGL(k_guide_train, dim3((uint32_t)ctx->num_cus), dim3(kBlock), lds, st, gs.d_theta, gs.d_wbf, d_rec,
-                       n, gs.d_grad, gs.d_acc, gs.loss_scale, guide_box(ctx));
-    if (gs.ev[1]) (void)hipEventRecord(gs.ev[1], st);
-    hipLaunchKernelGGL(k_guide_check, dim3(64), dim3(kBlock), 0, st, gs.d_grad, reinterpret_cast<uint32_t*>(gs.d_acc + 4));
-    float acc[8];
-    s = check_hip(ctx, hipMemcpyAsync(acc, gs.d_acc, sizeof(acc), hipMemcpyDeviceToHost, st), "guide acc read");
-    if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "guide train");
-    if (s != SPT_OK) return s;
-    uint32_t bad = 0;
-    std::memcpy(&bad, &acc[4], 4);
-    const float wsum = acc[0];
-    if (bad || !(wsum > 0.0f)) {
-        if (bad) gs.loss_scale *= 0.5f;
-        ++gs.skipped;
-    } else {
-        ++gs.step;
-        gs.trained += n;
-        gs.last_loss = acc[1];
-    }
-    if (gs.ev[0] && gs.ev[1]) {
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, gs.ev[0], gs.ev[1]) == hipSuccess) gs.kernel_ms += ms;
-    }
-    const float bc1 = 1.0f - std::pow(0.9f, (float)std::max<uint64_t>(gs.step, 1));
-    const float bc2 = 1.0f - std::pow(0.99f, (float)std::max<uint64_t>(gs.step, 1));
-    // a skipped step still clears the gradient buffer (flag set: no update)
-    if (!bad && !(wsum > 0.0f)) (void)hipMemsetD32Async((hipDeviceptr_t)(gs.d_acc + 4), 1, 1, st);
+    GuideCtl* ctl = gs.ctl();
+    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, st, ctl);
+    hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, &ctl->wsum);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (gs.event_pair(e0, e1)) (void)hipEventRecord(e0, st);
+    hipLaunchKernelGGL(k_guide_train, dim3((uint32_t)ctx->num_cus), dim3(kBlock), train_lds_bytes(), st, gs.d_theta,
+                       gs.d_wbf, d_rec, n, gs.d_grad, ctl, guide_box(ctx));
+    if (e1) (void)hipEventRecord(e1, st);
+    hipLaunchKernelGGL(k_guide_check, dim3(64), dim3(kBlock), 0, st, gs.d_grad, ctl);
     hipLaunchKernelGGL(k_guide_adam, dim3((kParams + kBlock - 1) / kBlock), dim3(kBlock), 0, st, gs.d_theta, gs.d_m,
-                       gs.d_v, gs.d_grad, gs.d_wbf, reinterpret_cast<const uint32_t*>(gs.d_acc + 4), gs.lr,
-                       1.0f / gs.loss_scale, bc1, bc2);
-    s = check_hip(ctx, hipStreamSynchronize(st), "guide adam");
-    if (s != SPT_OK) return s;
-    if (loss_out) *loss_out = acc[1];
+                       gs.d_v, gs.d_grad, gs.d_wbf, ctl, gs.lr);
+    hipLaunchKernelGGL(k_guide_post, dim3(1), dim3(64), 0, st, ctl, n);
+    spt_status s = check_hip(ctx, hipGetLastError(), "guide train launch");
+    if (s == SPT_OK && loss_out) {  // only a caller that wants the loss waits for it
+        GuideCtl h;
+        s = read_small(ctx, &h, ctl, sizeof(h), "guide step");
+        if (s == SPT_OK) *loss_out = h.loss;
+    }
     gs.train_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return SPT_OK;
+    return s;
 }
 
 spt_status guide_grad_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, float* grad_out, float* loss_out) {
@@ -262,21 +268,20 @@ spt_status guide_grad_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, float
     spt_status ls = allow_lds(ctx);
     if (ls != SPT_OK) return ls;
     hipStream_t st = ctx->stream;
-    spt_status s = check_hip(ctx, hipMemsetAsync(gs.d_acc, 0, 64, st), "guide acc");
-    if (s == SPT_OK) s = check_hip(ctx, hipMemsetAsync(gs.d_grad, 0, kParams * sizeof(float), st), "guide grad");
+    GuideCtl* ctl = gs.ctl();
+    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, st, ctl);
+    spt_status s = check_hip(ctx, hipMemsetAsync(gs.d_grad, 0, kParams * sizeof(float), st), "guide grad");
     if (s != SPT_OK) return s;
-    hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, gs.d_acc);
+    hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, &ctl->wsum);
     hipLaunchKernelGGL(k_guide_train, dim3((uint32_t)ctx->num_cus), dim3(kBlock), train_lds_bytes(), st, gs.d_theta,
-                       gs.d_wbf, d_rec, n, gs.d_grad, gs.d_acc, gs.loss_scale, guide_box(ctx));
-    float acc[8];
-    s = check_hip(ctx, hipMemcpyAsync(acc, gs.d_acc, sizeof(acc), hipMemcpyDeviceToHost, st), "guide acc read");
-    if (s == SPT_OK)
-        s = check_hip(ctx, hipMemcpyAsync(grad_out, gs.d_grad, kParams * sizeof(float), hipMemcpyDeviceToHost, st),
-                      "guide grad read");
-    if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "guide grad");
+                       gs.d_wbf, d_rec, n, gs.d_grad, ctl, guide_box(ctx));
+    s = check_hip(ctx, hipMemcpyAsync(grad_out, gs.d_grad, kParams * sizeof(float), hipMemcpyDeviceToHost, st),
+                  "guide grad read");
+    GuideCtl h;
+    if (s == SPT_OK) s = read_small(ctx, &h, ctl, sizeof(h), "guide grad");
     if (s != SPT_OK) return s;
-    for (int i = 0; i < kParams; ++i) grad_out[i] /= gs.loss_scale;
-    if (loss_out) *loss_out = acc[1];
+    for (int i = 0; i < kParams; ++i) grad_out[i] /= h.scale;
+    if (loss_out) *loss_out = h.loss;
     return check_hip(ctx, hipMemsetAsync(gs.d_grad, 0, kParams * sizeof(float), st), "guide grad clear");
 }
 
@@ -302,8 +307,12 @@ spt_status upload_theta(spt_ctx* ctx, const float* theta) {
     (void)hipMemset(gs.d_v, 0, kParams * sizeof(float));
     (void)hipMemset(gs.d_grad, 0, kParams * sizeof(float));
     hipLaunchKernelGGL(k_guide_tobf16, dim3((kParams + 255) / 256), dim3(256), 0, ctx->stream, gs.d_theta, gs.d_wbf);
-    gs.step = 0;
-    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "guide upload");
+    // fresh moments: step count and statistics restart; the loss scale is the configured one
+    GuideCtl c{};
+    c.scale = gs.loss_scale;
+    s = check_hip(ctx, hipMemcpyAsync(gs.ctl(), &c, sizeof(c), hipMemcpyHostToDevice, ctx->stream), "guide control");
+    if (s == SPT_OK) s = stream_wait(ctx, "guide upload");
+    return s;
 }
 
 }  // namespace
@@ -318,10 +327,13 @@ spt_status spt_guide_init(spt_ctx* ctx, const spt_guide_config* cfg) {
     c.loss_scale = 65536.0f;
     c.bsdf_fraction = 0.5f;
     c.batch = 262144;
+    c.steps_per_pass = 1;
     if (cfg) c = *cfg;
+    if (c.steps_per_pass == 0) c.steps_per_pass = 1;
     if (!(c.bsdf_fraction > 0.0f && c.bsdf_fraction < 1.0f) || c.batch == 0 || !(c.learning_rate > 0.0f) ||
-        !(c.loss_scale > 0.0f))
+        !(c.loss_scale > 0.0f) || c.steps_per_pass > 64 || (uint64_t)c.batch * c.steps_per_pass >= (1ull << 31))
         return set_error(ctx, SPT_ERR_INVALID, "guide config out of range");
+    const uint32_t cap = c.batch * c.steps_per_pass;
     GuideState& gs = ctx->guide;
     if (!gs.d_theta) {
         void* p = nullptr;
@@ -336,28 +348,27 @@ spt_status spt_guide_init(spt_ctx* ctx, const spt_guide_config* cfg) {
         gs.d_acc = gs.d_grad + kParams;
         gs.d_wbf = reinterpret_cast<unsigned short*>(gs.d_acc + 64);
     }
-    if (gs.records_cap < c.batch) {
+    if (gs.records_alloc < cap) {
         if (gs.d_records) (void)hipFree(gs.d_records);
         gs.d_records = nullptr;
-        spt_status s = check_hip(ctx, hipMalloc(&gs.d_records, (size_t)c.batch * kRecord * sizeof(float) + 256),
+        gs.records_alloc = 0;
+        spt_status s = check_hip(ctx, hipMalloc(&gs.d_records, (size_t)cap * kRecord * sizeof(float) + 256),
                                  "guide records");
         if (s != SPT_OK) return s;
-        gs.records_cap = c.batch;
+        gs.records_alloc = cap;
     }
+    gs.records_cap = cap;
     gs.lr = c.learning_rate;
     gs.loss_scale = c.loss_scale;
     gs.bsdf_fraction = c.bsdf_fraction;
     gs.batch = c.batch;
-    gs.skipped = 0;
     gs.train_ms = 0.0;
     gs.kernel_ms = 0.0;
-    gs.trained = 0;
     gs.candidates = 0;
     gs.rec_stride = 1;
+    gs.cand_per_path = 0.0;
     gs.collected = 0;
-    gs.last_loss = 0.0f;
-    for (hipEvent_t& e : gs.ev)
-        if (!e && hipEventCreate(&e) != hipSuccess) e = nullptr;
+    gs.settle_events();
     // Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero biases; output
     // layer scaled by 0.1 so the first mixtures are broad.
     std::vector<float> th(kParams, 0.0f);
@@ -418,22 +429,38 @@ spt_status spt_guide_train_collected(spt_ctx* ctx, float* loss) {
         return SPT_OK;
     }
     const uint32_t n = (uint32_t)std::min<uint64_t>(gs.collected, gs.records_cap);
-    spt_status s = guide_train_launch(ctx, gs.d_records, n, loss);
+    spt_status s = SPT_OK;
+    // one step per batch; a remainder shorter than half a batch joins the last step
+    for (uint32_t b0 = 0; b0 < n && s == SPT_OK;) {
+        uint32_t len = std::min(gs.batch, n - b0);
+        if (n - b0 - len < gs.batch / 2) len = n - b0;
+        const bool last = b0 + len >= n;
+        s = guide_train_launch(ctx, gs.d_records + (size_t)b0 * kRecord, len, last ? loss : nullptr);
+        b0 += len;
+    }
     gs.collected = 0;
     return s;
 }
 
 spt_status spt_guide_get_stats(spt_ctx* ctx, spt_guide_stats* out) {
     if (!ctx || !out) return SPT_ERR_INVALID;
-    const GuideState& gs = ctx->guide;
-    out->steps = gs.step;
-    out->skipped_steps = gs.skipped;
+    GuideState& gs = ctx->guide;
+    GuideCtl h{};
+    if (gs.ready) {
+        spt_status s = read_small(ctx, &h, gs.ctl(), sizeof(h), "guide stats");
+        if (s != SPT_OK) return s;
+        gs.settle_events();
+    } else {
+        h.scale = gs.loss_scale;
+    }
+    out->steps = h.steps;
+    out->skipped_steps = h.skipped;
     out->records = gs.collected;
-    out->last_loss = gs.last_loss;
-    out->loss_scale = gs.loss_scale;
+    out->last_loss = h.last_loss;
+    out->loss_scale = h.scale;
     out->train_ms = gs.train_ms;
     out->train_kernel_ms = gs.kernel_ms;
-    out->trained_records = gs.trained;
+    out->trained_records = h.trained;
     out->candidates = gs.candidates;
     out->record_stride = gs.rec_stride;
     out->pad = 0;

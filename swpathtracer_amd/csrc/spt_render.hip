@@ -799,6 +799,10 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
         }
     }
     const uint32_t check = graph_on ? std::max<uint32_t>(2, (tn.check_every + 1) & ~1u) : tn.check_every;
+    // With a small depth cap every bounce is issued without reading the live
+    // count (a bounce with no live path costs a few empty launches); without
+    // one the count is read every `check` bounces.
+    const bool poll = !(cap_depth >= 0 && cap_depth <= 32);
     for (uint64_t done = 0; done < total; done += cap) {
         const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - done);
         hipLaunchKernelGGL((k_generate<MULTI>), dim3(blocks_for(n)), dim3(kBlock), 0, st, a, buf[0], w, done, n);
@@ -816,23 +820,21 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
             }
             if (s != SPT_OK) break;
             if (cap_depth >= 0 && b >= (uint32_t)cap_depth) break;
-            if (b % check == 0) {
+            if (poll && b % check == 0) {
                 uint32_t live = 0;
-                s = check_hip(ctx, hipMemcpyAsync(&live, w.cnt + (b & 1), 4, hipMemcpyDeviceToHost, st), "count");
-                if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "bounce");
+                s = read_small(ctx, &live, w.cnt + (b & 1), 4, "bounce");
                 if (s != SPT_OK || live == 0) break;
             }
         }
         if (s != SPT_OK) break;
     }
     if (s != SPT_OK) {
-        (void)hipStreamSynchronize(st);
+        (void)stream_wait(ctx, "render abort");
         drop_graph();
         return s;
     }
     uint32_t cnt[20];  // [16]: guide records written, [17]: candidate vertices
-    s = check_hip(ctx, hipMemcpyAsync(cnt, w.cnt, sizeof(cnt), hipMemcpyDeviceToHost, st), "counters");
-    if (s == SPT_OK) s = check_hip(ctx, hipStreamSynchronize(st), "render");
+    s = read_small(ctx, cnt, w.cnt, sizeof(cnt), "render");
     drop_graph();
     if (s != SPT_OK) return s;
     stats.stream_overflows = cnt[2];
@@ -841,7 +843,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
         gs.collected = std::min<uint64_t>(cnt[16], a.rec_cap);
         gs.candidates = cnt[17];
         // next pass: record about one batch worth of candidates
-        gs.rec_stride = (uint32_t)std::max<uint64_t>(1, (cnt[17] + gs.batch - 1) / std::max<uint32_t>(gs.batch, 1));
+        gs.cand_per_path = total ? (double)cnt[17] / (double)total : 0.0;
     }
     for (size_t i = 0; i + 1 < gev.size(); i += 2) {
         float ms = 0.0f;
@@ -857,11 +859,11 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
         }
     }
     uint32_t flags = 0;
-    s = check_hip(ctx, hipMemcpy(&flags, ctx->d_flags, 4, hipMemcpyDeviceToHost), "flags");
+    s = read_small(ctx, &flags, ctx->d_flags, 4, "flags");
     if (s != SPT_OK) return s;
     if (fetch_stats) {
         unsigned long long f[2] = {0, 0};
-        s = check_hip(ctx, hipMemcpy(f, ctx->d_flags + 4, 16, hipMemcpyDeviceToHost), "fetch counters");
+        s = read_small(ctx, f, ctx->d_flags + 4, 16, "fetch counters");
         if (s != SPT_OK) return s;
         stats.node_fetches = f[0];
         stats.tri_fetches = f[1];
@@ -899,6 +901,11 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     a.bsdf_frac = ctx->guide.bsdf_fraction;
     a.records = ctx->guide.d_records;
     a.rec_cap = ctx->guide.records_cap;
+    // record about records_cap of this render's candidate vertices, estimated
+    // from the candidates per path of the previous training render
+    if (ctx->guide.cand_per_path > 0.0 && ctx->guide.records_cap > 0)
+        ctx->guide.rec_stride = (uint32_t)std::min<double>(
+            1 << 30, std::max(1.0, std::ceil(ctx->guide.cand_per_path * (double)total / ctx->guide.records_cap)));
     a.rec_stride = ctx->guide.rec_stride;
     const uint32_t keys = ctx->n_mats + 1;
     a.keys = (!(prm->flags & SPT_RENDER_NO_SORT) && keys <= kMaxSortKeys) ? keys : 0u;

@@ -93,6 +93,23 @@ def test_trace_edge_cases(device):
     assert len(t0) == 0
 
 
+def test_trace_non_finite_rays_miss(device):
+    """NaN or infinite ray components: every t comparison of the reference
+    fails (no hit); the persistent traversal rejects such rays at the root
+    instead of visiting every box, and agrees with the oracle's result."""
+    sc = scenes.displaced_mesh(64, 128)
+    _, orc = _setup(device, sc)
+    nan, inf = np.float32("nan"), np.float32("inf")
+    rays = np.array([[0, 0, 3, 0, 0, 0, -1, 0], [0, 0, 3, 0, nan, 0, -1, 0], [nan, 0, 3, 0, 0, 0, -1, 0],
+                     [0, 0, 3, 0, inf, 0, -1, 0], [0, 0, inf, 0, 0, 0, -1, 0], [0, 0, 3, 0, 0, nan, nan, 0]],
+                    np.float32)
+    rays = np.tile(rays, (4096, 1))
+    t_g, p_g = device.trace_host(rays)
+    t_c, p_c, _, _ = orc.trace(rays[:6])
+    assert p_g[0] != L.SPT_MISS and p_c[0] == p_g[0]
+    assert np.all(p_g.reshape(-1, 6)[:, 1:] == L.SPT_MISS) and np.all(p_c[1:] == L.SPT_MISS)
+
+
 def test_trace_root_leaf_scene(device):
     """A scene small enough that the root is a leaf (no split beats N)."""
     tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0], [0, 0, 1, 1, 0, 1, 0, 1, 1]], np.float32)

@@ -121,3 +121,12 @@ def test_bf16_rounding():
     assert r[0] == 1.0 and r[1] == 1.0  # tie to even
     assert r[2] == np.float32(1.0078125)
     assert abs(r[3] + 3.140625) < 1e-6
+
+
+def test_vmf_sample_limit_at_underflow():
+    """For large kappa e^{-2 kappa} underflows; u1 = 0 must give the limit
+    direction -mu (w = -1), not a NaN."""
+    mu = np.array([[0.0, 0.0, 1.0]])
+    for kappa in (50.0, 1e3, 1e4):
+        d = G.sample_vmf(mu, np.array([kappa]), np.array([0.0]), np.array([0.3]))
+        assert np.all(np.isfinite(d)) and np.allclose(d, [[0.0, 0.0, -1.0]], atol=1e-12)
