@@ -180,12 +180,16 @@ __device__ inline void trace_ray(const DevScene& sc, V3 o, V3 d, Stack& st, Hit&
 // Each wave loops until the ray pool is empty: idle lanes take the next rays
 // with one wave-aggregated atomic, then every lane runs an inner-node phase
 // (descend near-first, push the far child) followed by a leaf phase (test the
-// triangles of the leaf it stopped at, pop the next entry). Leaves are stack
-// entries too, encoded as kLeafBit | node << 1 | side, so a lane that reaches
-// a leaf waits in the leaf phase for the wave instead of branching alone; the
-// leaf's box (needed for the reference's hit-point-inside-leaf test) is reread
-// from its parent node, which is cache-resident by then.
-// Same candidate acceptance and tie rule as trace_ray<true, false>.
+// triangles of postponed and consecutive leaves). Leaves are stack entries
+// too, encoded as kLeafBit | node << 1 | side. The inner phase is
+// speculative (Aila and Laine, "Understanding the efficiency of ray traversal
+// on GPUs", 2009): a lane that meets its first leaf postpones it and keeps
+// descending until every lane holds a leaf, so lanes wait less for each
+// other. The leaf's box (needed for the reference's hit-point-inside-leaf
+// test) is reread from its parent node, which is cache-resident by then.
+// Same candidate acceptance and tie rule as trace_ray<true, false>: the
+// closest hit is the minimum of (t, leaf index), independent of visit order,
+// and boxes are only skipped when they start beyond the best t found so far.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kDone = 0xFFFFFFFFu;
@@ -207,6 +211,7 @@ struct PLane {
     V3 o, d, inv;
     Hit h;
     uint32_t cur;   // node index, leaf entry, or kDone
+    uint32_t leaf;  // postponed leaf entry, or kDone
     int sp;
     uint32_t ray;   // index in the batch (kDone: idle)
     uint32_t nodes, tris;  // FETCH counters
@@ -222,6 +227,7 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
     PLane<FETCH> L;
     L.ray = kDone;
     L.cur = kDone;
+    L.leaf = kDone;
     L.sp = 0;
     bool pool_empty = false;
     while (true) {
@@ -245,6 +251,7 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                     L.h.mat = 0;
                     L.h.u = L.h.v = 0.0f;
                     L.sp = 0;
+                    L.leaf = kDone;
                     if (FETCH) L.nodes = L.tris = 0;
                     float tn, tf;
                     // a ray with a non-finite component cannot hit anything (every
@@ -261,66 +268,85 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
             if (pool_empty) break;
             continue;
         }
-        // ---- inner-node phase
-        while (L.ray != kDone && L.cur != kDone && !(L.cur & kLeafBit)) {
-            const uint32_t node = L.cur;
-            const float4* np = sc.nodes + 4 * (size_t)node;
-            const float4 a = np[0], b = np[1], c = np[2];
-            const uint4 r = reinterpret_cast<const uint4*>(np)[3];
-            if (FETCH) ++L.nodes;
-            float tnl = 0.0f, tfl = 0.0f, tnr = 0.0f, tfr = 0.0f;
-            const bool hl = slab(L.o, L.inv, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, tnl, tfl) && !(tnl > L.h.t) &&
-                            !(tfl < 0.0f);
-            const bool hr = slab(L.o, L.inv, V3{b.z, b.w, c.x}, V3{c.y, c.z, c.w}, tnr, tfr) && !(tnr > L.h.t) &&
-                            !(tfr < 0.0f);
-            const uint32_t el = r.y ? (kLeafBit | (node << 1)) : r.x;
-            const uint32_t er = r.w ? (kLeafBit | (node << 1) | 1u) : r.z;
-            if (hl && hr) {
-                const bool rfirst = tnr < tnl;
-                if (L.sp >= (int)st.cap) {
-                    overflow = true;
-                    L.cur = kDone;
-                    L.sp = 0;
-                    break;
-                }
-                st.put(L.sp++, rfirst ? el : er);
-                L.cur = rfirst ? er : el;
-            } else if (hl) {
-                L.cur = el;
-            } else if (hr) {
-                L.cur = er;
-            } else {
-                L.cur = (L.sp > 0) ? st.get(--L.sp) : kDone;
-            }
-        }
-        // ---- leaf phase
-        while (L.ray != kDone && L.cur != kDone && (L.cur & kLeafBit)) {
-            uint32_t first, cnt;
-            V3 bmn, bmx;
-            if (R.leaf) {
-                first = R.tri0;
-                cnt = R.leaf;
-                bmn = R.mn;
-                bmx = R.mx;
-            } else {
-                // whole 64-B record (cache-resident), then select the side in registers
-                const uint32_t node = (L.cur & ~kLeafBit) >> 1;
+        // ---- inner-node phase, speculative: a lane that meets its first leaf
+        // postpones it and keeps descending; the phase ends when no lane is
+        // still searching for a leaf (or none has inner work left)
+        while (true) {
+            if (L.ray != kDone && L.cur != kDone && !(L.cur & kLeafBit)) {
+                const uint32_t node = L.cur;
                 const float4* np = sc.nodes + 4 * (size_t)node;
                 const float4 a = np[0], b = np[1], c = np[2];
                 const uint4 r = reinterpret_cast<const uint4*>(np)[3];
-                const bool right = (L.cur & 1u) != 0;
-                bmn = right ? V3{b.z, b.w, c.x} : V3{a.x, a.y, a.z};
-                bmx = right ? V3{c.y, c.z, c.w} : V3{a.w, b.x, b.y};
-                first = right ? r.z : r.x;
-                cnt = right ? r.w : r.y;
+                if (FETCH) ++L.nodes;
+                float tnl = 0.0f, tfl = 0.0f, tnr = 0.0f, tfr = 0.0f;
+                const bool hl = slab(L.o, L.inv, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, tnl, tfl) &&
+                                !(tnl > L.h.t) && !(tfl < 0.0f);
+                const bool hr = slab(L.o, L.inv, V3{b.z, b.w, c.x}, V3{c.y, c.z, c.w}, tnr, tfr) &&
+                                !(tnr > L.h.t) && !(tfr < 0.0f);
+                const uint32_t el = r.y ? (kLeafBit | (node << 1)) : r.x;
+                const uint32_t er = r.w ? (kLeafBit | (node << 1) | 1u) : r.z;
+                if (hl && hr) {
+                    const bool rfirst = tnr < tnl;
+                    if (L.sp >= (int)st.cap) {
+                        overflow = true;
+                        L.cur = kDone;
+                        L.leaf = kDone;
+                        L.sp = 0;
+                    } else {
+                        st.put(L.sp++, rfirst ? el : er);
+                        L.cur = rfirst ? er : el;
+                    }
+                } else if (hl) {
+                    L.cur = el;
+                } else if (hr) {
+                    L.cur = er;
+                } else {
+                    L.cur = (L.sp > 0) ? st.get(--L.sp) : kDone;
+                }
+                if (L.cur != kDone && (L.cur & kLeafBit) && L.leaf == kDone) {
+                    L.leaf = L.cur;
+                    L.cur = (L.sp > 0) ? st.get(--L.sp) : kDone;
+                }
             }
-            uint32_t dummy_t = 0, dummy_i = 0;
-            leaf_test<false, false>(sc, L.o, L.d, first, cnt, bmn, bmx, L.h, dummy_t, dummy_i);
-            if (FETCH) L.tris += cnt;
-            L.cur = (L.sp > 0) ? st.get(--L.sp) : kDone;
+            const bool inner = L.ray != kDone && L.cur != kDone && !(L.cur & kLeafBit);
+            if (!__any(inner && L.leaf == kDone)) break;
+        }
+        // ---- leaf phase: the postponed leaf, then consecutive leaf entries
+        while (true) {
+            if (L.ray != kDone && L.leaf == kDone && L.cur != kDone && (L.cur & kLeafBit)) {
+                L.leaf = L.cur;
+                L.cur = (L.sp > 0) ? st.get(--L.sp) : kDone;
+            }
+            const bool has = L.ray != kDone && L.leaf != kDone;
+            if (!__any(has)) break;
+            if (has) {
+                uint32_t first, cnt;
+                V3 bmn, bmx;
+                if (R.leaf) {
+                    first = R.tri0;
+                    cnt = R.leaf;
+                    bmn = R.mn;
+                    bmx = R.mx;
+                } else {
+                    // whole 64-B record (cache-resident), then select the side in registers
+                    const uint32_t node = (L.leaf & ~kLeafBit) >> 1;
+                    const float4* np = sc.nodes + 4 * (size_t)node;
+                    const float4 a = np[0], b = np[1], c = np[2];
+                    const uint4 r = reinterpret_cast<const uint4*>(np)[3];
+                    const bool right = (L.leaf & 1u) != 0;
+                    bmn = right ? V3{b.z, b.w, c.x} : V3{a.x, a.y, a.z};
+                    bmx = right ? V3{c.y, c.z, c.w} : V3{a.w, b.x, b.y};
+                    first = right ? r.z : r.x;
+                    cnt = right ? r.w : r.y;
+                }
+                uint32_t dummy_t = 0, dummy_i = 0;
+                leaf_test<false, false>(sc, L.o, L.d, first, cnt, bmn, bmx, L.h, dummy_t, dummy_i);
+                if (FETCH) L.tris += cnt;
+                L.leaf = kDone;
+            }
         }
         // ---- finished rays
-        if (L.ray != kDone && L.cur == kDone) {
+        if (L.ray != kDone && L.cur == kDone && L.leaf == kDone) {
             put_hit(L.ray, L.h, FETCH ? L.nodes : 0u, FETCH ? L.tris : 0u);
             L.ray = kDone;
         }
