@@ -91,7 +91,7 @@ def closest_hit_bench(dev, mesh, n_rays, reps=5):
     nodes = d_nodes.download(np.uint32, n_rays).astype(np.float64)
     tris = d_tris.download(np.uint32, n_rays).astype(np.float64)
     hits = d_hits.download(np.uint32, (n_rays, 2))
-    byts = 32 * n_rays + 8 * n_rays + 64 * nodes.sum() + 48 * tris.sum()
+    byts = 32 * n_rays + 8 * n_rays + 128 * nodes.sum() + 48 * tris.sum()
     return {"rays": n_rays, "ms": ms, "grays_per_s": n_rays / (ms * 1e-3) / 1e9,
             "mean_nodes": nodes.mean(), "mean_tris": tris.mean(), "hit_frac": float((hits[:, 1] != L.SPT_MISS).mean()),
             "fetch_gbs": byts / (ms * 1e-3) / 1e9}
@@ -372,7 +372,7 @@ def main():
     per_ray_nodes = fs["node_fetches"] / fs["rays"]
     per_ray_tris = fs["tri_fetches"] / fs["rays"]
     # per extension ray: queue index 4 + origin 16 + direction 16 in, hit 16 + sort key 4 out
-    bytes_per_ray = 56 + 64 * per_ray_nodes + 48 * per_ray_tris
+    bytes_per_ray = 56 + 128 * per_ray_nodes + 48 * per_ray_tris
     achieved = bytes_per_ray * rays / (ext_ms * 1e-3) / 1e9
     img = film.download(np.float32, (cam.height, cam.width, 3)) / (args.steps * sps)
 

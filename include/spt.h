@@ -153,8 +153,8 @@ enum {
     SPT_RENDER_PER_PATH = 1u << 2,  /* write each path's radiance into d_path_rgb */
     SPT_RENDER_NO_SORT = 1u << 3,   /* skip the material sort between stages */
     SPT_RENDER_GUIDED = 1u << 4,    /* sample with the learned guide (spt_guide_*) */
-    SPT_RENDER_FETCH_STATS = 1u << 5, /* count node / triangle record fetches of the
-                                         extension rays (slower traversal variant) */
+    SPT_RENDER_FETCH_STATS = 1u << 5, /* count 128-B node / 48-B triangle record fetches
+                                         of the extension rays (slower traversal variant) */
     SPT_RENDER_TIME_KERNELS = 1u << 6 /* time every extend launch with HIP events */
 };
 
@@ -259,7 +259,8 @@ enum {
                                 returns its counters (sbvh.cpp:478-510) */
     SPT_TRACE_UNPRUNED = 2,  /* like COUNT without writing counters */
     SPT_TRACE_FETCH = 3      /* CLOSEST, and d_tests/d_isects receive the number of
-                                64-B node and 48-B triangle records each ray fetched */
+                                128-B four-wide node records and 48-B triangle
+                                records each ray fetched */
 };
 
 /* d_rays: n rays; d_hits: n hits. In SPT_TRACE_COUNT mode d_tests/d_isects

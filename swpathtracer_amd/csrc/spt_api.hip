@@ -85,9 +85,12 @@ namespace {
 
 void free_scene(spt_ctx* ctx) {
     if (ctx->d_nodes) (void)hipFree(ctx->d_nodes);
+    if (ctx->d_nodes4) (void)hipFree(ctx->d_nodes4);
     if (ctx->d_tris) (void)hipFree(ctx->d_tris);
     if (ctx->d_shading) (void)hipFree(ctx->d_shading);
     ctx->d_nodes = nullptr;
+    ctx->d_nodes4 = nullptr;
+    ctx->n_nodes4 = 0;
     ctx->d_tris = nullptr;
     ctx->d_shading = nullptr;
     ctx->n_nodes = ctx->n_tris = 0;
@@ -155,6 +158,93 @@ void scene_arrays(const spt_bvh* bvh, const float* verts, const float* normals, 
 }
 
 uint32_t tree_stack_cap(const spt_node64* nodes, uint32_t n) { return tree_depth(nodes, n) + 1; }
+
+namespace {
+
+struct Entry4 {
+    float lo[3], hi[3];
+    uint32_t ref, cnt;  // cnt > 0: leaf (first triangle, count); 0: binary node index
+};
+
+float area4(const Entry4& e) {
+    const float dx = e.hi[0] - e.lo[0], dy = e.hi[1] - e.lo[1], dz = e.hi[2] - e.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+void side(const spt_node64& nd, int s, Entry4& e) {
+    const float* lo = s == 0 ? nd.lmin : nd.rmin;
+    const float* hi = s == 0 ? nd.lmax : nd.rmax;
+    for (int k = 0; k < 3; ++k) {
+        e.lo[k] = lo[k];
+        e.hi[k] = hi[k];
+    }
+    e.ref = s == 0 ? nd.lref : nd.rref;
+    e.cnt = s == 0 ? nd.lcnt : nd.rcnt;
+}
+
+}  // namespace
+
+uint32_t collapse4(const spt_node64* nodes, uint32_t n, std::vector<Node4>& out) {
+    out.clear();
+    if (n == 0) return 1;
+    // work list: binary node, the Node4 slot it fills, its depth
+    struct Item {
+        uint32_t bin, slot, depth;
+    };
+    std::vector<Item> work{{0u, 0u, 1u}};
+    out.emplace_back();
+    uint32_t max_depth = 1;
+    while (!work.empty()) {
+        const Item j = work.back();
+        work.pop_back();
+        max_depth = std::max(max_depth, j.depth);
+        // the binary node's two children, then the inner child with the
+        // largest surface area is opened until there are four
+        Entry4 e[4];
+        int m = 0;
+        side(nodes[j.bin], 0, e[m++]);
+        side(nodes[j.bin], 1, e[m++]);
+        while (m < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int i = 0; i < m; ++i)
+                if (e[i].cnt == 0 && area4(e[i]) > ba) {
+                    ba = area4(e[i]);
+                    best = i;
+                }
+            if (best < 0) break;
+            const spt_node64& nd = nodes[e[best].ref];
+            side(nd, 0, e[best]);
+            side(nd, 1, e[m++]);
+        }
+        Node4 q;
+        for (int i = 0; i < 4; ++i) {
+            if (i < m) {
+                q.lox[i] = e[i].lo[0];
+                q.hix[i] = e[i].hi[0];
+                q.loy[i] = e[i].lo[1];
+                q.hiy[i] = e[i].hi[1];
+                q.loz[i] = e[i].lo[2];
+                q.hiz[i] = e[i].hi[2];
+                q.count[i] = e[i].cnt;
+                if (e[i].cnt > 0) {
+                    q.child[i] = e[i].ref;
+                } else {
+                    q.child[i] = (uint32_t)out.size();
+                    out.emplace_back();
+                    work.push_back({e[i].ref, q.child[i], j.depth + 1});
+                }
+            } else {
+                q.lox[i] = q.loy[i] = q.loz[i] = 0.0f;
+                q.hix[i] = q.hiy[i] = q.hiz[i] = 0.0f;
+                q.child[i] = kNoChild;
+                q.count[i] = 0;
+            }
+        }
+        out[j.slot] = q;
+    }
+    return 3 * max_depth + 1;
+}
 
 }  // namespace spt
 
@@ -295,7 +385,10 @@ spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_no
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     free_scene(ctx);
+    std::vector<Node4> n4;
+    const uint32_t cap4 = collapse4(nodes, n_nodes, n4);
     spt_status s = upload(ctx, &ctx->d_nodes, nodes, (size_t)n_nodes * sizeof(spt_node64), "nodes");
+    if (s == SPT_OK) s = upload(ctx, &ctx->d_nodes4, n4.data(), n4.size() * sizeof(Node4), "four-wide nodes");
     if (s == SPT_OK) s = upload(ctx, &ctx->d_tris, tris, (size_t)n_tris * sizeof(spt_tri48), "triangles");
     std::vector<float> flat;
     if (s == SPT_OK) {
@@ -326,6 +419,8 @@ spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_no
     std::memcpy(ctx->root_box, root_box, sizeof(ctx->root_box));
     ctx->root_leaf = n_nodes == 0 ? n_tris : 0;
     ctx->stack_cap = tree_depth(nodes, n_nodes) + 1;
+    ctx->n_nodes4 = (uint32_t)n4.size();
+    ctx->stack_cap4 = cap4;
     ctx->max_mat = 0;
     for (uint32_t i = 0; i < n_tris; ++i) ctx->max_mat = std::max(ctx->max_mat, tris[i].pad[0]);
     ctx->has_scene = true;

@@ -16,8 +16,21 @@ constexpr int kBlock = 256;       // threads per block of the ray kernels (4 wav
 constexpr int kLdsStack = 16;     // traversal stack entries per lane kept in LDS
 constexpr uint32_t kNoHit = SPT_MISS;
 
+// Four-wide node of the throughput traversal, collapsed from the binary
+// tree (children are binary-tree nodes or leaves two levels down, with their
+// own padded boxes): child boxes as six float4 (lo.x of the 4 children, hi.x,
+// lo.y, hi.y, lo.z, hi.z), then the child words and leaf counts.
+struct Node4 {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    uint32_t child[4];  // inner: Node4 index; leaf: first triangle; unused: kNoChild
+    uint32_t count[4];  // leaf: triangle count (> 0); inner or unused: 0
+};
+static_assert(sizeof(Node4) == 128, "Node4 layout");
+constexpr uint32_t kNoChild = 0xFFFFFFFFu;
+
 // Scene as the kernels see it.
 struct DevScene {
+    const float4* nodes4;   // n_nodes4 x 8 float4 (Node4), persistent traversal
     const float4* nodes;    // n_nodes x 4 float4 (spt_node64)
     const float4* tris;     // n_tris x 3 float4 (spt_tri48; .pad[0] = material id)
     const float4* shading;  // n_tris x 3 float4: n0 n1 n2, material, smooth flag, pad
@@ -28,6 +41,7 @@ struct DevScene {
     uint32_t n_nodes;
     uint32_t n_mats;
     uint32_t stack_cap;  // traversal stack entries a lane may need (inner-node depth + 1)
+    uint32_t stack_cap4; // same for the four-wide tree (3 per level + 1)
 };
 
 // Closest-hit result of one ray.
@@ -206,9 +220,45 @@ __device__ inline Root scene_root(const DevScene& sc) {
     return Root{V3{sc.rmin[0], sc.rmin[1], sc.rmin[2]}, V3{sc.rmax[0], sc.rmax[1], sc.rmax[2]}, sc.root_leaf, 0u, 0u};
 }
 
+// Reciprocal for the conservative box tests: a zero component gives a large
+// finite value, so (lo - o) / d products never form inf - inf.
+__device__ inline float safe_rcp(float d) {
+    return (fabsf(d) > 1e-30f) ? 1.0f / d : copysignf(1e30f, d);
+}
+
+// One child of a Node4 against the ray: key = entry distance (+inf when the
+// box is missed, lies behind the origin or beyond the best hit t_best, or the
+// slot is unused), entry = stack entry of the child. The test is one FMA per
+// slab plane and IEEE min/max (a NaN product keeps the box): conservative, it
+// may visit a box the reference's exact slab test skips but never skips one
+// holding a closer candidate, since every box is padded 1e-3 around its
+// triangles (acceleration/sbvh.cpp:409-412).
+__device__ inline void child_test(V3 inv, V3 oi, float t_best, float lox, float hix, float loy, float hiy, float loz,
+                                  float hiz, uint32_t child, uint32_t cnt, uint32_t node, uint32_t slot, float& key,
+                                  uint32_t& entry) {
+    const float t0x = fmaf(lox, inv.x, oi.x), t1x = fmaf(hix, inv.x, oi.x);
+    const float t0y = fmaf(loy, inv.y, oi.y), t1y = fmaf(hiy, inv.y, oi.y);
+    const float t0z = fmaf(loz, inv.z, oi.z), t1z = fmaf(hiz, inv.z, oi.z);
+    const float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+    const float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), t_best));
+    key = (child != kNoChild && tmin <= tmax) ? tmin : __builtin_inff();
+    entry = cnt ? (kLeafBit | (node << 2) | slot) : child;
+}
+
+__device__ inline void cswap(float k[4], uint32_t e[4], int a, int b) {
+    const bool sw = k[b] < k[a];
+    const float ka = k[a], kb = k[b];
+    const uint32_t ea = e[a], eb = e[b];
+    k[a] = sw ? kb : ka;
+    k[b] = sw ? ka : kb;
+    e[a] = sw ? eb : ea;
+    e[b] = sw ? ea : eb;
+}
+
 template <bool FETCH>
 struct PLane {
     V3 o, d, inv;
+    V3 inv4, oi;    // box tests of the four-wide tree: finite reciprocal, -o * inv4
     Hit h;
     uint32_t cur;   // node index, leaf entry, or kDone
     uint32_t leaf;  // postponed leaf entry, or kDone
@@ -245,6 +295,8 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                     L.ray = i;
                     get_ray(i, L.o, L.d, R);
                     L.inv = V3{1.0f / L.d.x, 1.0f / L.d.y, 1.0f / L.d.z};
+                    L.inv4 = V3{safe_rcp(L.d.x), safe_rcp(L.d.y), safe_rcp(L.d.z)};
+                    L.oi = V3{-L.o.x * L.inv4.x, -L.o.y * L.inv4.y, -L.o.z * L.inv4.z};
                     L.h.t = __builtin_inff();
                     L.h.idx = kNoHit;
                     L.h.prim = kNoHit;
@@ -274,34 +326,39 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
         while (true) {
             if (L.ray != kDone && L.cur != kDone && !(L.cur & kLeafBit)) {
                 const uint32_t node = L.cur;
-                const float4* np = sc.nodes + 4 * (size_t)node;
-                const float4 a = np[0], b = np[1], c = np[2];
-                const uint4 r = reinterpret_cast<const uint4*>(np)[3];
+                const float4* np = sc.nodes4 + 8 * (size_t)node;
+                const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+                const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+                const uint4 cn = reinterpret_cast<const uint4*>(np)[7];
                 if (FETCH) ++L.nodes;
-                float tnl = 0.0f, tfl = 0.0f, tnr = 0.0f, tfr = 0.0f;
-                const bool hl = slab(L.o, L.inv, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, tnl, tfl) &&
-                                !(tnl > L.h.t) && !(tfl < 0.0f);
-                const bool hr = slab(L.o, L.inv, V3{b.z, b.w, c.x}, V3{c.y, c.z, c.w}, tnr, tfr) &&
-                                !(tnr > L.h.t) && !(tfr < 0.0f);
-                const uint32_t el = r.y ? (kLeafBit | (node << 1)) : r.x;
-                const uint32_t er = r.w ? (kLeafBit | (node << 1) | 1u) : r.z;
-                if (hl && hr) {
-                    const bool rfirst = tnr < tnl;
-                    if (L.sp >= (int)st.cap) {
+                float k[4];
+                uint32_t e[4];
+                child_test(L.inv4, L.oi, L.h.t, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x, cn.x, node, 0, k[0], e[0]);
+                child_test(L.inv4, L.oi, L.h.t, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y, cn.y, node, 1, k[1], e[1]);
+                child_test(L.inv4, L.oi, L.h.t, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z, cn.z, node, 2, k[2], e[2]);
+                child_test(L.inv4, L.oi, L.h.t, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w, cn.w, node, 3, k[3], e[3]);
+                // nearest first: sort the four (entry, entry distance) pairs, misses (+inf) last
+                cswap(k, e, 0, 1);
+                cswap(k, e, 2, 3);
+                cswap(k, e, 0, 2);
+                cswap(k, e, 1, 3);
+                cswap(k, e, 1, 2);
+                const float inf = __builtin_inff();
+                if (k[0] == inf) {
+                    L.cur = (L.sp > 0) ? st.get(--L.sp) : kDone;
+                } else {
+                    const int need = (int)(k[1] < inf) + (int)(k[2] < inf) + (int)(k[3] < inf);
+                    if (L.sp + need > (int)st.cap) {
                         overflow = true;
                         L.cur = kDone;
                         L.leaf = kDone;
                         L.sp = 0;
                     } else {
-                        st.put(L.sp++, rfirst ? el : er);
-                        L.cur = rfirst ? er : el;
+                        if (k[3] < inf) st.put(L.sp++, e[3]);
+                        if (k[2] < inf) st.put(L.sp++, e[2]);
+                        if (k[1] < inf) st.put(L.sp++, e[1]);
+                        L.cur = e[0];
                     }
-                } else if (hl) {
-                    L.cur = el;
-                } else if (hr) {
-                    L.cur = er;
-                } else {
-                    L.cur = (L.sp > 0) ? st.get(--L.sp) : kDone;
                 }
                 if (L.cur != kDone && (L.cur & kLeafBit) && L.leaf == kDone) {
                     L.leaf = L.cur;
@@ -328,16 +385,13 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                     bmn = R.mn;
                     bmx = R.mx;
                 } else {
-                    // whole 64-B record (cache-resident), then select the side in registers
-                    const uint32_t node = (L.leaf & ~kLeafBit) >> 1;
-                    const float4* np = sc.nodes + 4 * (size_t)node;
-                    const float4 a = np[0], b = np[1], c = np[2];
-                    const uint4 r = reinterpret_cast<const uint4*>(np)[3];
-                    const bool right = (L.leaf & 1u) != 0;
-                    bmn = right ? V3{b.z, b.w, c.x} : V3{a.x, a.y, a.z};
-                    bmx = right ? V3{c.y, c.z, c.w} : V3{a.w, b.x, b.y};
-                    first = right ? r.z : r.x;
-                    cnt = right ? r.w : r.y;
+                    // the leaf's box and range from its Node4 (cache-resident by now)
+                    const uint32_t node = (L.leaf & ~kLeafBit) >> 2, slot = L.leaf & 3u;
+                    const float* f = reinterpret_cast<const float*>(sc.nodes4 + 8 * (size_t)node) + slot;
+                    bmn = V3{f[0], f[8], f[16]};
+                    bmx = V3{f[4], f[12], f[20]};
+                    first = __float_as_uint(f[24]);
+                    cnt = __float_as_uint(f[28]);
                 }
                 uint32_t dummy_t = 0, dummy_i = 0;
                 leaf_test<false, false>(sc, L.o, L.d, first, cnt, bmn, bmx, L.h, dummy_t, dummy_i);
@@ -430,10 +484,11 @@ struct BatchTile {
 // Device arrays and per-scene jobs of a batch (spt_batch_*).
 struct BatchData {
     float4* d_nodes = nullptr;
+    float4* d_nodes4 = nullptr;
     float4* d_tris = nullptr;
     float4* d_shading = nullptr;
     spt_material* d_mats = nullptr;
-    uint32_t n_nodes = 0, n_tris = 0, n_mats = 0, stack_cap = 0;
+    uint32_t n_nodes = 0, n_tris = 0, n_mats = 0, stack_cap = 0, stack_cap4 = 0;
     std::vector<Job> jobs;  // host copy: sampling, outputs, roots
     Job* d_jobs = nullptr;
     BatchTile* d_tiles = nullptr;
@@ -532,6 +587,8 @@ struct spt_ctx {
     int num_cus = 256;
     // scene
     float4* d_nodes = nullptr;
+    float4* d_nodes4 = nullptr;
+    uint32_t n_nodes4 = 0, stack_cap4 = 0;
     float4* d_tris = nullptr;
     float4* d_shading = nullptr;
     spt_material* d_mats = nullptr;
@@ -565,6 +622,8 @@ struct spt_ctx {
     spt::DevScene scene() const {
         spt::DevScene s;
         s.nodes = d_nodes;
+        s.nodes4 = d_nodes4;
+        s.stack_cap4 = stack_cap4;
         s.tris = d_tris;
         s.shading = d_shading;
         s.mats = d_mats;
@@ -604,6 +663,9 @@ spt_tuning resolve_tuning(const spt_tuning& t);
 void scene_arrays(const spt_bvh* bvh, const float* verts, const float* normals, const uint32_t* mat_ids,
                   std::vector<spt_tri48>& tris, std::vector<float>& sh);
 uint32_t tree_stack_cap(const spt_node64* nodes, uint32_t n);
+// Collapses a pre-order pair-node tree into four-wide nodes (pre-order, root
+// first); returns the stack entries the persistent traversal needs.
+uint32_t collapse4(const spt_node64* nodes, uint32_t n, std::vector<Node4>& out);
 spt_status make_job(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* prm, float* d_rgb32, double* d_rgb64,
                     float* d_path_rgb, Job& jb, uint64_t& total);
 // A multi-scene render (spt_batch_*): every job of the batch, or with

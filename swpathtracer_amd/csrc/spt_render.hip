@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, PathBuf A, Work
         atomicAdd(reinterpret_cast<unsigned long long*>(w.cnt + 10), (unsigned long long)n);
         atomicAdd(reinterpret_cast<unsigned long long*>(w.cnt + 12), 1ull);
     }
-    Stack st(lds + threadIdx.x, spill + gtid, gthreads, a.sc.stack_cap);
+    Stack st(lds + threadIdx.x, spill + gtid, gthreads, a.sc.stack_cap4);
     bool overflow = false;
     unsigned long long my_nodes = 0, my_tris = 0;
     if (FETCH) {
@@ -693,7 +693,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
     const uint32_t ext_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)(per_cu * ctx->num_cus));
     const uint32_t shade_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * tn.shade_blocks_per_cu);
     spt_status s = ensure_spill(ctx, (size_t)ext_grid * kBlock *
-                                         (a.sc.stack_cap > kLdsStack ? a.sc.stack_cap - kLdsStack : 1));
+                                         (a.sc.stack_cap4 > kLdsStack ? a.sc.stack_cap4 - kLdsStack : 1));
     if (s != SPT_OK) return s;
     hipStream_t st = ctx->stream;
     const bool timed = (a.flags & SPT_RENDER_TIME_KERNELS) != 0;
@@ -994,6 +994,8 @@ spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div) {
 
     RenderArgs a{};
     a.sc.nodes = b.d_nodes;
+    a.sc.nodes4 = b.d_nodes4;
+    a.sc.stack_cap4 = b.stack_cap4;
     a.sc.tris = b.d_tris;
     a.sc.shading = b.d_shading;
     a.sc.mats = b.d_mats;

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_p(DevScene sc, const float4* _
     __shared__ uint32_t lds[kLdsStack * kBlock];
     const uint32_t gthreads = gridDim.x * kBlock;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    Stack st(lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap);
+    Stack st(lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap4);
     bool overflow = false;
     trace_persistent<FETCH>(
         sc, n, flags + 8, st, overflow,
@@ -134,7 +134,7 @@ spt_status launch_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_h
                                                     : reinterpret_cast<const void*>(&k_trace_p<false>);
         const uint32_t grid = persistent_grid(ctx, kern);
         spt_status s =
-            ensure_spill(ctx, (size_t)grid * kBlock * (ctx->stack_cap > kLdsStack ? ctx->stack_cap - kLdsStack : 1));
+            ensure_spill(ctx, (size_t)grid * kBlock * (ctx->stack_cap4 > kLdsStack ? ctx->stack_cap4 - kLdsStack : 1));
         if (s != SPT_OK) return s;
         s = check_hip(ctx, hipMemsetAsync(ctx->d_flags + 8, 0, 4, ctx->stream), "ray counter");
         if (s != SPT_OK) return s;
