@@ -274,6 +274,7 @@ def main():
     ap.add_argument("--spp-per-step", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-closest-hit", action="store_true")
+    ap.add_argument("--no-sort", action="store_true", help="skip the material sort between stages")
     ap.add_argument("--rings", type=int, default=512)
     ap.add_argument("--segments", type=int, default=1024)
     ap.add_argument("--guide", action="store_true", help="also run the guided vs plain equal-RMSE experiment")
@@ -340,7 +341,7 @@ def main():
         p = integ.params()
         p.spp_begin = (i * sps) % spp_total
         p.spp_count = sps
-        p.flags = flags
+        p.flags = flags | (L.SPT_RENDER_NO_SORT if args.no_sort else 0)
         dev.render(cam, p, film.ptr, None, None)
 
     for i in range(args.warmup):

@@ -329,7 +329,8 @@ typedef struct spt_tuning {
     uint32_t tile_pixels;          /* multi-scene calls: pixels per generation tile (default 4096) */
     uint32_t tile_samples;         /* multi-scene calls: samples per tile (default 16) */
     uint32_t check_every;          /* bounces between reads of the live count (default 4) */
-    uint32_t pad;
+    uint32_t refill_lanes;         /* a traversal wave fetches new rays once this many of
+                                      its 64 lanes are idle (default 16) */
 } spt_tuning;
 spt_status spt_set_tuning(spt_ctx* ctx, const spt_tuning* t);
 spt_status spt_get_tuning(spt_ctx* ctx, spt_tuning* t);

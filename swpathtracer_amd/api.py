@@ -352,13 +352,13 @@ class Device:
     def get_tuning(self) -> dict:
         t = L.Tuning()
         self._check(self.lib.spt_get_tuning(self.h, C.byref(t)))
-        return {f: getattr(t, f) for f, _ in L.Tuning._fields_ if f != "pad"}
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_}
 
     def autotune(self, cam: L.Camera, integ: "Integrator", sample_paths: int = 8 << 20) -> dict:
         p = integ.params()
         t = L.Tuning()
         self._check(self.lib.spt_autotune(self.h, C.byref(cam), C.byref(p), sample_paths, C.byref(t)))
-        return {f: getattr(t, f) for f, _ in L.Tuning._fields_ if f != "pad"}
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_}
 
     # learned guide -----------------------------------------------------
     def guide_init(self, seed: int = 1, lr: float = 5e-3, loss_scale: float = 65536.0, bsdf_fraction: float = 0.5,
@@ -521,7 +521,7 @@ class Batch:
     def autotune(self, sample_div: int = 16) -> dict:
         t = L.Tuning()
         self.dev._check(self.lib.spt_batch_autotune(self.dev.h, self.h, sample_div, C.byref(t)))
-        return {f: getattr(t, f) for f, _ in L.Tuning._fields_ if f != "pad"}
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_}
 
     def stats(self):
         st = L.BatchStats()

@@ -257,8 +257,8 @@ __device__ inline void cswap(float k[4], uint32_t e[4], int a, int b) {
 
 template <bool FETCH>
 struct PLane {
-    V3 o, d, inv;
-    V3 inv4, oi;    // box tests of the four-wide tree: finite reciprocal, -o * inv4
+    V3 o, d;
+    V3 inv4, oi;    // box tests: finite reciprocal, -o * inv4
     Hit h;
     uint32_t cur;   // node index, leaf entry, or kDone
     uint32_t leaf;  // postponed leaf entry, or kDone
@@ -271,7 +271,7 @@ struct PLane {
 // starts as the single scene's root); PutHit(i, h, nodes, tris) stores its result.
 template <bool FETCH, typename GetRay, typename PutHit>
 __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t* ray_counter, Stack& st,
-                                        bool& overflow, GetRay get_ray, PutHit put_hit) {
+                                        bool& overflow, uint32_t refill, GetRay get_ray, PutHit put_hit) {
     Root R = scene_root(sc);
     const uint64_t lt = __lanemask_lt();
     PLane<FETCH> L;
@@ -283,7 +283,7 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
     while (true) {
         // ---- refill idle lanes
         const uint64_t idle = __ballot(L.ray == kDone);
-        if (!pool_empty && __popcll(idle) >= 16) {
+        if (!pool_empty && (uint32_t)__popcll(idle) >= refill) {
             uint32_t base = 0;
             const int leader = __ffsll((unsigned long long)idle) - 1;
             if (__lane_id() == (unsigned)leader) base = atomicAdd(ray_counter, (uint32_t)__popcll(idle));
@@ -294,7 +294,6 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                 if (i < n) {
                     L.ray = i;
                     get_ray(i, L.o, L.d, R);
-                    L.inv = V3{1.0f / L.d.x, 1.0f / L.d.y, 1.0f / L.d.z};
                     L.inv4 = V3{safe_rcp(L.d.x), safe_rcp(L.d.y), safe_rcp(L.d.z)};
                     L.oi = V3{-L.o.x * L.inv4.x, -L.o.y * L.inv4.y, -L.o.z * L.inv4.z};
                     L.h.t = __builtin_inff();
@@ -305,11 +304,14 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                     L.sp = 0;
                     L.leaf = kDone;
                     if (FETCH) L.nodes = L.tris = 0;
-                    float tn, tf;
                     // a ray with a non-finite component cannot hit anything (every
                     // t comparison fails) but would pass every box test: a miss
                     const float sum = L.o.x + L.o.y + L.o.z + L.d.x + L.d.y + L.d.z;
-                    if (!slab(L.o, L.inv, R.mn, R.mx, tn, tf) || !(sum - sum == 0.0f))
+                    float troot;
+                    uint32_t unused;
+                    child_test(L.inv4, L.oi, __builtin_inff(), R.mn.x, R.mx.x, R.mn.y, R.mx.y, R.mn.z, R.mx.z, 0u, 0u,
+                               0u, 0u, troot, unused);
+                    if (troot == __builtin_inff() || !(sum - sum == 0.0f))
                         L.cur = kDone;
                     else
                         L.cur = R.leaf ? kLeafBit : R.node0;

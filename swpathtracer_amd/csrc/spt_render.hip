@@ -58,6 +58,7 @@ struct RenderArgs {
     uint32_t* job_done;         // paths finished per job
     uint64_t* job_clock;        // [0] start, [1 + j] finish of job j (device wall clock)
     uint32_t flags;
+    uint32_t refill;  // idle lanes that trigger a ray refill in the traversal
     uint32_t keys;  // sort keys (n_mats + 1), 0 = no sort
     // learned guide
     uint32_t guided, train;
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, PathBuf A, Work
         __syncthreads();
     }
     trace_persistent<FETCH>(
-        a.sc, n, w.cnt + 8, st, overflow,
+        a.sc, n, w.cnt + 8, st, overflow, a.refill,
         [&](uint32_t i, V3& o, V3& d, Root& root) {
             const float4 o4 = A.o[i], d4 = A.d[i];
             o = V3{o4.x, o4.y, o4.z};
@@ -563,6 +564,7 @@ spt_tuning resolve_tuning(const spt_tuning& t) {
     if (r.tile_pixels == 0) r.tile_pixels = 4096;
     if (r.tile_samples == 0) r.tile_samples = 16;
     if (r.check_every == 0) r.check_every = 4;
+    if (r.refill_lanes == 0) r.refill_lanes = 16;
     return r;
 }
 
@@ -670,6 +672,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
     if ((size_t)(base - reinterpret_cast<char*>(ctx->d_paths)) > ctx->paths_bytes)
         return set_error(ctx, SPT_ERR_NOMEM, "path state layout exceeds its allocation");
     a.rec_count = w.cnt + 16;
+    a.refill = std::min<uint32_t>(64, std::max<uint32_t>(1, tn.refill_lanes));
     if (a.guided) {
         const size_t need = (size_t)cap * guide::kOut * sizeof(float);
         if (need > ctx->guide.gout_cap) {

@@ -38,14 +38,14 @@ template <bool FETCH>
 __global__ __launch_bounds__(kBlock) void k_trace_p(DevScene sc, const float4* __restrict__ rays, uint32_t n,
                                                     uint2* __restrict__ hits, uint32_t* __restrict__ nodes_out,
                                                     uint32_t* __restrict__ tris_out, uint32_t* __restrict__ spill,
-                                                    uint32_t* __restrict__ flags) {
+                                                    uint32_t* __restrict__ flags, uint32_t refill) {
     __shared__ uint32_t lds[kLdsStack * kBlock];
     const uint32_t gthreads = gridDim.x * kBlock;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     Stack st(lds + threadIdx.x, spill + gtid, gthreads, sc.stack_cap4);
     bool overflow = false;
     trace_persistent<FETCH>(
-        sc, n, flags + 8, st, overflow,
+        sc, n, flags + 8, st, overflow, refill,
         [&](uint32_t i, V3& o, V3& d, Root&) {
             const float4 ro = rays[2 * (size_t)i], rd = rays[2 * (size_t)i + 1];
             o = V3{ro.x, ro.y, ro.z};
@@ -141,12 +141,13 @@ spt_status launch_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_h
         DevScene sc = ctx->scene();
         const float4* rays = reinterpret_cast<const float4*>(d_rays);
         uint2* hits = reinterpret_cast<uint2*>(d_hits);
+        const uint32_t refill = std::min<uint32_t>(64, std::max<uint32_t>(1, resolve_tuning(ctx->tuning).refill_lanes));
         if (mode == SPT_TRACE_FETCH)
             hipLaunchKernelGGL(k_trace_p<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, (uint32_t)n, hits,
-                               d_tests, d_isects, ctx->d_spill, ctx->d_flags);
+                               d_tests, d_isects, ctx->d_spill, ctx->d_flags, refill);
         else
             hipLaunchKernelGGL(k_trace_p<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, (uint32_t)n,
-                               hits, nullptr, nullptr, ctx->d_spill, ctx->d_flags);
+                               hits, nullptr, nullptr, ctx->d_spill, ctx->d_flags, refill);
         return check_hip(ctx, hipGetLastError(), "trace launch");
     }
     const uint32_t grid = grid_for(ctx, n);
