@@ -277,17 +277,17 @@ def main():
     ap.add_argument("--no-sort", action="store_true", help="skip the material sort between stages")
     ap.add_argument("--rings", type=int, default=512)
     ap.add_argument("--segments", type=int, default=1024)
-    ap.add_argument("--guide", action="store_true", help="also run the guided vs plain equal-RMSE experiment")
+    ap.add_argument("--no-guide", action="store_true", help="skip the guided vs plain equal-RMSE experiment")
     ap.add_argument("--guide-only", action="store_true")
-    ap.add_argument("--batch", action="store_true", help="also run the 64-scene batched render")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched render")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--guide-full", action="store_true", help="keep rendering after equal RMSE (full curve)")
     ap.add_argument("--guide-res", type=int, nargs=2, default=(800, 600))
-    ap.add_argument("--oracle-spp", type=int, default=16384)
+    ap.add_argument("--oracle-spp", type=int, default=8192, help="spp of the FP64-film reference image")
     ap.add_argument("--guide-lr", type=float, default=5e-3)
-    ap.add_argument("--guide-warmup", type=int, default=32, help="1-spp training passes before the counted image")
+    ap.add_argument("--guide-warmup", type=int, default=16, help="1-spp training passes before the counted image")
     ap.add_argument("--guide-pass-spp", type=int, default=16, help="spp of each counted guided pass (a square)")
-    ap.add_argument("--guide-steps", type=int, default=4, help="Adam steps per training pass")
+    ap.add_argument("--guide-steps", type=int, default=8, help="Adam steps per training pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -383,9 +383,11 @@ def main():
         ch = closest_hit_bench(dev, mesh, 16_777_216)
         log(f"closest hit: {ch}")
         extra["closest_hit_incoherent"] = {k: round(v, 4) if isinstance(v, float) else v for k, v in ch.items()}
-    if args.guide:
+    film.free()
+    dev.close()  # each experiment below opens its own context
+    if not args.no_guide:
         extra["guide_experiment"] = guide_experiment(args)
-    if args.batch:
+    if not args.no_batch:
         extra["batch"] = batch_bench(args)
     cpu = None
     if not args.no_cpu:

@@ -299,6 +299,7 @@ void spt_destroy(spt_ctx* ctx) {
     if (ctx->guide.d_theta) (void)hipFree(ctx->guide.d_theta);
     if (ctx->guide.d_records) (void)hipFree(ctx->guide.d_records);
     if (ctx->guide.d_gout) (void)hipFree(ctx->guide.d_gout);
+    if (ctx->guide.d_scratch) (void)hipFree(ctx->guide.d_scratch);
     ctx->guide.destroy_events();
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
     if (ctx->d_guidable) (void)hipFree(ctx->d_guidable);

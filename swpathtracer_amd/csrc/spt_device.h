@@ -544,7 +544,13 @@ struct GuideState {
     uint32_t rec_stride = 1;      // 1 in rec_stride candidate vertices is recorded
     double cand_per_path = 0.0;   // candidate vertices per path of the last training render
     uint64_t candidates = 0;      // candidate vertices of the last training render
-    double kernel_ms = 0.0;       // device time of k_guide_train (settled steps)
+    double kernel_ms = 0.0;       // device time of the training kernels (settled steps)
+    // training scratch, feature-major (feature f of sample s at f * rows + s):
+    // x^T bf16 [32], h1..h3^T bf16 [64] each, dy^T f32 [48], dz3..dz1^T bf16
+    // [64] each, then per-split partial gradients f32 [splits][params]
+    char* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    uint32_t scratch_rows = 0;
     std::vector<hipEvent_t> ev_free, ev_pending;  // event pairs around training kernels
 
     GuideCtl* ctl() const { return reinterpret_cast<GuideCtl*>(d_acc); }

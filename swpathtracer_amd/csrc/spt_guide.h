@@ -65,18 +65,6 @@ __device__ inline void encode(V3 p, V3 n, V3 wo, const float* bmin, const float*
     f[31] = 1.0f;
 }
 
-// Per-wave LDS tiles (bf16 bits). Rows = 16 queries.
-struct WaveTiles {
-    unsigned short x[16 * kEnc];
-    unsigned short h1[16 * kHid];
-    unsigned short h2[16 * kHid];
-    unsigned short h3[16 * kHid];
-    float y[16 * kOutP];
-    unsigned short dy[16 * kOutP];
-    unsigned short da[16 * kHid];
-    unsigned short db[16 * kHid];
-};
-
 // Block-shared weights: bf16 copies of W1..W4 and fp32 biases.
 struct SharedWeights {
     unsigned short w1[kHid * kEnc];
@@ -125,41 +113,6 @@ __device__ inline bf16x4 ld4s(const unsigned short* p, int stride) {
 //   A (16 rows x 16 k):  A[r][4g + j]         j = 0..3
 //   B (16 k x 16 cols):  B[4g + j][r]
 //   D (16 x 16, f32):    D[4g + i][r]         i = 0..3
-// Dense layer on one 16-row tile: out[row][c] = act(sum_k in[row][k] W[c][k] + b[c])
-// in: LDS [16][K] bf16; W: LDS [N][K] bf16 (row-major out x in).
-template <int K, int N, bool RELU>
-__device__ inline void dense_fwd(const unsigned short* in, const unsigned short* W, const float* b, unsigned short* out_bf,
-                                 float* out_f) {
-    const int l = __lane_id(), r = l & 15, g = l >> 4;
-#pragma unroll
-    for (int t = 0; t < N / 16; ++t) {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = 0; s < K / 16; ++s) {
-            const bf16x4 a = ld4(in + r * K + 16 * s + 4 * g);
-            const bf16x4 w = ld4(W + (16 * t + r) * K + 16 * s + 4 * g);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc, 0, 0, 0);
-        }
-        const int col = 16 * t + r;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float v = acc[i] + b[col];
-            if (RELU) v = fmaxf(v, 0.0f);
-            const int row = 4 * g + i;
-            if (out_bf) out_bf[row * N + col] = f2bf(v);
-            if (out_f) out_f[row * N + col] = v;
-        }
-    }
-}
-
-// Full forward of one tile: t.x -> t.h1 -> t.h2 -> t.h3 -> t.y (f32).
-__device__ inline void mlp_tile_fwd(const SharedWeights& s, WaveTiles& t) {
-    dense_fwd<kEnc, kHid, true>(t.x, s.w1, s.b1, t.h1, nullptr);
-    dense_fwd<kHid, kHid, true>(t.h1, s.w2, s.b2, t.h2, nullptr);
-    dense_fwd<kHid, kHid, true>(t.h2, s.w3, s.b3, t.h3, nullptr);
-    dense_fwd<kHid, kOutP, false>(t.h3, s.w4, s.b4, nullptr, t.y);
-}
-
 // ---------------------------------------------------------------------------
 // Register-resident forward pass ("column" form): the layer is computed as
 // out^T = W * in^T, weights as the MFMA A operand (from LDS) and 16 samples as
@@ -362,54 +315,6 @@ __device__ inline float loss_grad(const float* y, V3 d, float W, float* dy) {
     }
     for (int i = kOut; i < kOutP; ++i) dy[i] = 0.0f;
     return -W * logq;
-}
-
-// dW[m][n] += sum_rows dOut[row][m] * In[row][n]  (MFMA over the 16 rows),
-// added into an fp32 LDS accumulator with ds_add_f32.
-template <int M, int N>
-__device__ inline void dense_wgrad(const unsigned short* dout, const unsigned short* in, float* dW) {
-    const int l = __lane_id(), r = l & 15, g = l >> 4;
-#pragma unroll
-    for (int mt = 0; mt < M / 16; ++mt) {
-        // A[i][k] = dout[k][16 mt + i]: rows of the tile are the reduction axis
-        const bf16x4 a = ld4s(dout + (4 * g) * M + 16 * mt + r, M);
-#pragma unroll
-        for (int nt = 0; nt < N / 16; ++nt) {
-            const bf16x4 b = ld4s(in + (4 * g) * N + 16 * nt + r, N);
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) atomicAdd(&dW[(16 * mt + 4 * g + i) * N + 16 * nt + r], acc[i]);
-        }
-    }
-}
-
-// dIn[row][n] = (sum_m dOut[row][m] W[m][n]) * (in[row][n] > 0); bias grad of
-// this layer's input side is accumulated by the caller.
-template <int M, int N>
-__device__ inline void dense_igrad(const unsigned short* dout, const unsigned short* W, const unsigned short* act,
-                                   unsigned short* din, float* dbias) {
-    const int l = __lane_id(), r = l & 15, g = l >> 4;
-#pragma unroll
-    for (int t = 0; t < N / 16; ++t) {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = 0; s < M / 16; ++s) {
-            const bf16x4 a = ld4(dout + r * M + 16 * s + 4 * g);
-            const bf16x4 w = ld4s(W + (16 * s + 4 * g) * N + 16 * t + r, N);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc, 0, 0, 0);
-        }
-        const int col = 16 * t + r;
-        float bsum = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 4 * g + i;
-            const float v = (bf2f(act[row * N + col]) > 0.0f) ? acc[i] : 0.0f;
-            din[row * N + col] = f2bf(v);
-            bsum += v;
-        }
-        if (dbias) atomicAdd(&dbias[col], bsum);
-    }
 }
 
 }  // namespace guide

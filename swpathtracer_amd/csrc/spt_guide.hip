@@ -56,85 +56,272 @@ __global__ __launch_bounds__(kBlock) void k_guide_wsum(const float* __restrict__
     if (__lane_id() == 0) atomicAdd(acc, s);
 }
 
-// One training step's gradient: forward, per-row KL-loss gradient (weights
-// normalised by the batch mean, averaged over the batch, times the loss
-// scale), backward; weight gradients summed in LDS then added to grad (fp32).
-// Reads ctl->wsum and ctl->scale, adds the step's loss to ctl->loss.
-__global__ __launch_bounds__(kBlock) void k_guide_train(const float* __restrict__ theta,
-                                                        const unsigned short* __restrict__ wbf,
-                                                        const float* __restrict__ rec, uint32_t n,
-                                                        float* __restrict__ grad, GuideCtl* __restrict__ ctl,
-                                                        GuideBox box) {
+// ---------------------------------------------------------------------------
+// Training step in four stages over a feature-major scratch (feature f of
+// sample s at f * rows + s): forward + loss gradient, backward through the
+// activations, weight gradients as split-K MFMA products, reduction.
+// ---------------------------------------------------------------------------
+struct TrainScratch {
+    unsigned short* xT;   // [32][rows]
+    unsigned short* h1T;  // [64][rows]
+    unsigned short* h2T;
+    unsigned short* h3T;
+    float* dyT;           // [48][rows], rows 40..47 stay zero
+    unsigned short* dz3T; // [64][rows]
+    unsigned short* dz2T;
+    unsigned short* dz1T;
+    float* partial;       // [splits][kParams]
+    uint32_t rows;
+};
+
+__device__ inline bf16x4 cvt4(float a, float b, float c, float d) {
+    bf16x4 r;
+    r[0] = (short)f2bf(a);
+    r[1] = (short)f2bf(b);
+    r[2] = (short)f2bf(c);
+    r[3] = (short)f2bf(d);
+    return r;
+}
+
+// feature-major bf16 column: features f0 + i (i = 0..3) of sample s
+__device__ inline bf16x4 ld_col4(const unsigned short* T, uint32_t rows, uint32_t f0, uint32_t s) {
+    bf16x4 r;
+    r[0] = (short)T[(size_t)f0 * rows + s];
+    r[1] = (short)T[(size_t)(f0 + 1) * rows + s];
+    r[2] = (short)T[(size_t)(f0 + 2) * rows + s];
+    r[3] = (short)T[(size_t)(f0 + 3) * rows + s];
+    return r;
+}
+
+__device__ inline void st_col4(unsigned short* T, uint32_t rows, uint32_t f0, uint32_t s, bf16x4 v) {
+    T[(size_t)f0 * rows + s] = (unsigned short)v[0];
+    T[(size_t)(f0 + 1) * rows + s] = (unsigned short)v[1];
+    T[(size_t)(f0 + 2) * rows + s] = (unsigned short)v[2];
+    T[(size_t)(f0 + 3) * rows + s] = (unsigned short)v[3];
+}
+
+// A operand of W^T for the backward pass: A[r][4g + j] = W[16 ot + 4g + j][16 kt + r]
+__device__ inline bf16x4 ld_wT(const unsigned short* W, int K, int ot, int kt) {
+    const int l = __lane_id(), r = l & 15, g = l >> 4;
+    return ld4s(W + (16 * ot + 4 * g) * K + 16 * kt + r, K);
+}
+
+// Stage 1: encode, forward (activations stored feature-major), and each
+// record's loss gradient w.r.t. the 48 raw outputs, scaled by loss_scale / n
+// (rows n..rows16: zero gradient). One block = 256 records, lane = record.
+__global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ theta,
+                                                      const unsigned short* __restrict__ wbf,
+                                                      const float* __restrict__ rec, uint32_t n, uint32_t n16,
+                                                      GuideCtl* __restrict__ ctl, GuideBox box, TrainScratch ts) {
     extern __shared__ float dyn[];
     SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
-    float* dW = reinterpret_cast<float*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
-    WaveTiles* tiles = reinterpret_cast<WaveTiles*>(dW + kParams);
+    WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
+    float* ystage = reinterpret_cast<float*>(stages + kBlock / 64);  // [4 waves][64][kOut]
     load_weights(sw, wbf, theta);
-    for (int i = threadIdx.x; i < kParams; i += blockDim.x) dW[i] = 0.0f;
+    const int wave = threadIdx.x >> 6, l = __lane_id(), r = l & 15, g = l >> 4;
+    WaveStage& st = stages[wave];
+    float* ys = ystage + wave * 64 * kOut;
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t wbase = blockIdx.x * kBlock + wave * 64;
+    float f[kEnc];
+    if (s < n) {
+        const float* p = rec + (size_t)s * kRecord;
+        encode(V3{p[0], p[1], p[2]}, V3{p[3], p[4], p[5]}, V3{p[6], p[7], p[8]}, box.bmin, box.bext, f);
+    } else {
+        for (int i = 0; i < kEnc; ++i) f[i] = 0.0f;
+    }
+    stage_row(st, f);
+    if (s < n16)
+        for (int i = 0; i < kEnc; ++i) ts.xT[(size_t)i * ts.rows + s] = f2bf(f[i]);
     __syncthreads();
-    const int wave = threadIdx.x >> 6, l = __lane_id();
-    WaveTiles& t = tiles[wave];
+    // forward, column form, 16 records at a time; activations out to scratch
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t sq = wbase + 16 * q + r;
+        const bool keep = sq < n16;
+        const unsigned short* row = st.x + (16 * q + r) * kStageRow;
+        const bf16x4 x[2] = {ld4(row + 4 * g), ld4(row + 16 + 4 * g)};
+        bf16x4 ha[4], hb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            f32x4 acc = bias4(sw.b1 + 16 * t + 4 * g);
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw.w1 + (16 * t + r) * kEnc + 16 * k + 4 * g), x[k],
+                                                                acc, 0, 0, 0);
+            ha[t] = pack_relu(acc);
+            if (keep) st_col4(ts.h1T, ts.rows, 16 * t + 4 * g, sq, ha[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            f32x4 acc = bias4(sw.b2 + 16 * t + 4 * g);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw.w2 + (16 * t + r) * kHid + 16 * k + 4 * g), ha[k],
+                                                                acc, 0, 0, 0);
+            hb[t] = pack_relu(acc);
+            if (keep) st_col4(ts.h2T, ts.rows, 16 * t + 4 * g, sq, hb[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            f32x4 acc = bias4(sw.b3 + 16 * t + 4 * g);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw.w3 + (16 * t + r) * kHid + 16 * k + 4 * g), hb[k],
+                                                                acc, 0, 0, 0);
+            ha[t] = pack_relu(acc);
+            if (keep) st_col4(ts.h3T, ts.rows, 16 * t + 4 * g, sq, ha[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            f32x4 acc = bias4(sw.b4 + 16 * t + 4 * g);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw.w4 + (16 * t + r) * kHid + 16 * k + 4 * g), ha[k],
+                                                                acc, 0, 0, 0);
+            if (16 * t + 4 * g < kOut)
+                for (int i = 0; i < 4; ++i) ys[(16 * q + r) * kOut + 16 * t + 4 * g + i] = acc[i];
+        }
+    }
+    __syncthreads();
+    // loss gradient of this lane's record
     const float wmean = fmaxf(ctl->wsum / (float)n, 1e-30f);
     const float scale = ctl->scale / (float)n;
     float loss = 0.0f;
-    const uint32_t ntiles = (n + 15) / 16;
-    for (uint32_t base = blockIdx.x * (kBlock / 64); base < ntiles; base += gridDim.x * (kBlock / 64)) {
-        const uint32_t tile = base + wave;
-        const bool act = tile < ntiles;
-        if (act && l < 16) {
-            const uint32_t row = tile * 16 + l;
-            float f[kEnc];
-            if (row < n) {
-                const float* r = rec + (size_t)row * kRecord;
-                encode(V3{r[0], r[1], r[2]}, V3{r[3], r[4], r[5]}, V3{r[6], r[7], r[8]}, box.bmin, box.bext, f);
-            } else {
-                for (int i = 0; i < kEnc; ++i) f[i] = 0.0f;
-            }
-            for (int i = 0; i < kEnc; ++i) t.x[l * kEnc + i] = f2bf(f[i]);
+    if (s < n16) {
+        float dy[kOutP];
+        if (s < n) {
+            const float* p = rec + (size_t)s * kRecord;
+            loss = loss_grad(ys + l * kOut, V3{p[9], p[10], p[11]}, p[12] / wmean, dy);
+        } else {
+            for (int i = 0; i < kOutP; ++i) dy[i] = 0.0f;
         }
-        __syncthreads();
-        if (act) mlp_tile_fwd(sw, t);
-        __syncthreads();
-        if (act && l < 16) {
-            const uint32_t row = tile * 16 + l;
-            float dy[kOutP];
-            if (row < n) {
-                const float* r = rec + (size_t)row * kRecord;
-                const float W = r[12] / wmean;
-                loss += loss_grad(&t.y[l * kOutP], V3{r[9], r[10], r[11]}, W, dy);
-            } else {
-                for (int i = 0; i < kOutP; ++i) dy[i] = 0.0f;
-            }
-            for (int i = 0; i < kOutP; ++i) {
-                const float v = dy[i] * scale;
-                t.dy[l * kOutP + i] = f2bf(v);
-                if (v != 0.0f) atomicAdd(&dW[kB4 + i], v);
-            }
-        }
-        __syncthreads();
-        if (act) {
-            dense_wgrad<kOutP, kHid>(t.dy, t.h3, dW + kW4);
-            dense_igrad<kOutP, kHid>(t.dy, sw.w4, t.h3, t.da, dW + kB3);
-        }
-        __syncthreads();
-        if (act) {
-            dense_wgrad<kHid, kHid>(t.da, t.h2, dW + kW3);
-            dense_igrad<kHid, kHid>(t.da, sw.w3, t.h2, t.db, dW + kB2);
-        }
-        __syncthreads();
-        if (act) {
-            dense_wgrad<kHid, kHid>(t.db, t.h1, dW + kW2);
-            dense_igrad<kHid, kHid>(t.db, sw.w2, t.h1, t.da, dW + kB1);
-        }
-        __syncthreads();
-        if (act) dense_wgrad<kHid, kEnc>(t.da, t.x, dW + kW1);
-        __syncthreads();
+        for (int o = 0; o < kOut; ++o) ts.dyT[(size_t)o * ts.rows + s] = dy[o] * scale;
     }
     for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
     if (l == 0) atomicAdd(&ctl->loss, loss / (float)n);
+}
+
+// Stage 2: gradients w.r.t. the hidden pre-activations, dz3 = (W4^T dy) . [h3 > 0],
+// dz2 = (W3^T dz3) . [h2 > 0], dz1 = (W2^T dz2) . [h1 > 0], column form, 16
+// records per wave per iteration; stored feature-major as bf16.
+__global__ __launch_bounds__(kBlock) void k_train_bwd(const float* __restrict__ theta,
+                                                      const unsigned short* __restrict__ wbf, uint32_t n16,
+                                                      TrainScratch ts) {
+    extern __shared__ float dyn[];
+    SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
+    load_weights(sw, wbf, theta);
     __syncthreads();
-    for (int i = threadIdx.x; i < kParams; i += blockDim.x)
-        if (dW[i] != 0.0f) atomicAdd(&grad[i], dW[i]);
+    const int l = __lane_id(), r = l & 15, g = l >> 4;
+    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), nw = gridDim.x * (kBlock / 64);
+    for (uint32_t base = wave * 16; base < n16; base += nw * 16) {
+        const uint32_t s = base + r;
+        bf16x4 dy[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const float* c = ts.dyT + (size_t)(16 * t + 4 * g) * ts.rows + s;
+            dy[t] = cvt4(c[0], c[ts.rows], c[2 * (size_t)ts.rows], c[3 * (size_t)ts.rows]);
+        }
+        bf16x4 za[4], zb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {  // dz3: K = 48 outputs
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_wT(sw.w4, kHid, k, t), dy[k], acc, 0, 0, 0);
+            const bf16x4 h = ld_col4(ts.h3T, ts.rows, 16 * t + 4 * g, s);
+            for (int i = 0; i < 4; ++i)
+                if (bf2f((unsigned short)h[i]) <= 0.0f) acc[i] = 0.0f;
+            za[t] = cvt4(acc[0], acc[1], acc[2], acc[3]);
+            st_col4(ts.dz3T, ts.rows, 16 * t + 4 * g, s, za[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {  // dz2
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_wT(sw.w3, kHid, k, t), za[k], acc, 0, 0, 0);
+            const bf16x4 h = ld_col4(ts.h2T, ts.rows, 16 * t + 4 * g, s);
+            for (int i = 0; i < 4; ++i)
+                if (bf2f((unsigned short)h[i]) <= 0.0f) acc[i] = 0.0f;
+            zb[t] = cvt4(acc[0], acc[1], acc[2], acc[3]);
+            st_col4(ts.dz2T, ts.rows, 16 * t + 4 * g, s, zb[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {  // dz1
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_wT(sw.w2, kHid, k, t), zb[k], acc, 0, 0, 0);
+            const bf16x4 h = ld_col4(ts.h1T, ts.rows, 16 * t + 4 * g, s);
+            for (int i = 0; i < 4; ++i)
+                if (bf2f((unsigned short)h[i]) <= 0.0f) acc[i] = 0.0f;
+            st_col4(ts.dz1T, ts.rows, 16 * t + 4 * g, s, cvt4(acc[0], acc[1], acc[2], acc[3]));
+        }
+    }
+}
+
+// 4 consecutive samples s0 .. s0 + 3 of feature f (feature-major bf16 / f32)
+__device__ inline bf16x4 ld_row4(const unsigned short* T, uint32_t rows, uint32_t f, uint32_t s0) {
+    const uint2 v = *reinterpret_cast<const uint2*>(T + (size_t)f * rows + s0);
+    bf16x4 r;
+    r[0] = (short)(v.x & 0xFFFFu);
+    r[1] = (short)(v.x >> 16);
+    r[2] = (short)(v.y & 0xFFFFu);
+    r[3] = (short)(v.y >> 16);
+    return r;
+}
+
+// Stage 3: weight and bias gradients of one layer over one slice of samples:
+// dW[o][k] = sum_s dz[o][s] in[k][s] as MFMA with the samples as the
+// reduction axis (K), db[o] = sum_s dz[o][s]; written to partial[split].
+// Block = (layer, split); wave w takes output tile w.
+__global__ __launch_bounds__(kBlock) void k_train_wgrad(TrainScratch ts, uint32_t n16, uint32_t span) {
+    const int layer = blockIdx.x & 3;  // 0: W1, 1: W2, 2: W3, 3: W4
+    const uint32_t split = blockIdx.x >> 2;
+    const int ot = threadIdx.x >> 6, l = __lane_id(), r = l & 15, g = l >> 4;
+    const int n_ot = layer == 3 ? 3 : 4, n_kt = layer == 0 ? 2 : 4, K = layer == 0 ? kEnc : kHid;
+    if (ot >= n_ot) return;
+    const unsigned short* dz = layer == 0 ? ts.dz1T : layer == 1 ? ts.dz2T : ts.dz3T;
+    const unsigned short* in = layer == 0 ? ts.xT : layer == 1 ? ts.h1T : layer == 2 ? ts.h2T : ts.h3T;
+    const uint32_t s_begin = split * span, s_end = min(n16, s_begin + span);
+    f32x4 acc[4];
+    for (int k = 0; k < 4; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    for (uint32_t s0 = s_begin; s0 < s_end; s0 += 16) {
+        bf16x4 a;
+        if (layer == 3) {
+            const float4 v = *reinterpret_cast<const float4*>(ts.dyT + (size_t)(16 * ot + r) * ts.rows + s0 + 4 * g);
+            a = cvt4(v.x, v.y, v.z, v.w);
+            bsum += (v.x + v.y) + (v.z + v.w);
+        } else {
+            a = ld_row4(dz, ts.rows, 16 * ot + r, s0 + 4 * g);
+            bsum += (bf2f((unsigned short)a[0]) + bf2f((unsigned short)a[1])) +
+                    (bf2f((unsigned short)a[2]) + bf2f((unsigned short)a[3]));
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < n_kt)
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ld_row4(in, ts.rows, 16 * k + r, s0 + 4 * g),
+                                                                   acc[k], 0, 0, 0);
+    }
+    float* out = ts.partial + (size_t)split * kParams;
+    const int wofs = layer == 0 ? kW1 : layer == 1 ? kW2 : layer == 2 ? kW3 : kW4;
+    const int bofs = layer == 0 ? kB1 : layer == 1 ? kB2 : layer == 2 ? kB3 : kB4;
+    for (int k = 0; k < n_kt; ++k)
+        for (int i = 0; i < 4; ++i) out[wofs + (16 * ot + 4 * g + i) * K + 16 * k + r] = acc[k][i];
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+    if (g == 0) out[bofs + 16 * ot + r] = bsum;
+}
+
+// Stage 4: grad = sum of the split partials (grad holds the scaled gradient).
+__global__ __launch_bounds__(kBlock) void k_train_reduce(const float* __restrict__ partial, uint32_t splits,
+                                                         float* __restrict__ grad) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (uint32_t)kParams) return;
+    float acc = 0.0f;
+    for (uint32_t sp = 0; sp < splits; ++sp) acc += partial[(size_t)sp * kParams + i];
+    grad[i] = acc;
 }
 
 // Start of a step: clears the per-step fields of the control block.
@@ -194,17 +381,19 @@ __global__ void k_guide_tobf16(const float* __restrict__ theta, unsigned short* 
 }
 
 size_t eval_lds_bytes() { return sizeof(SharedWeights) + sizeof(WaveStage) * (kBlock / 64); }
-size_t train_lds_bytes() { return sizeof(SharedWeights) + sizeof(float) * kParams + sizeof(WaveTiles) * (kBlock / 64); }
+size_t fwd_lds_bytes() {
+    return sizeof(SharedWeights) + sizeof(WaveStage) * (kBlock / 64) + sizeof(float) * 64 * kOut * (kBlock / 64);
+}
 
-// The training and evaluation kernels use more than the default 64 KiB of
+// The inference and forward kernels use more than the default 64 KiB of
 // dynamic LDS (gfx950 has 160 KiB per CU).
 spt_status allow_lds(spt_ctx* ctx) {
     static bool done = false;
     if (done) return SPT_OK;
     spt_status s = check_hip(ctx,
-                             hipFuncSetAttribute(reinterpret_cast<const void*>(&k_guide_train),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)train_lds_bytes()),
-                             "train LDS size");
+                             hipFuncSetAttribute(reinterpret_cast<const void*>(&k_train_fwd),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)fwd_lds_bytes()),
+                             "forward LDS size");
     if (s == SPT_OK)
         s = check_hip(ctx,
                       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_guide_eval),
@@ -212,6 +401,72 @@ spt_status allow_lds(spt_ctx* ctx) {
                       "eval LDS size");
     done = (s == SPT_OK);
     return s;
+}
+
+// Scratch for n16 records (grown on demand; the dy rows 40..47 stay zero).
+spt_status train_scratch(spt_ctx* ctx, uint32_t n16, uint32_t splits, TrainScratch& ts) {
+    GuideState& gs = ctx->guide;
+    const uint32_t rows = (n16 + 63) & ~63u;
+    const size_t per_row = 2 * kEnc + 3 * 2 * kHid + 4 * kOutP + 3 * 2 * kHid;
+    const size_t need = (size_t)rows * per_row + (size_t)splits * kParams * sizeof(float) + 1024;
+    if (rows > gs.scratch_rows || need > gs.scratch_bytes) {
+        spt_status s = stream_wait(ctx, "guide scratch");
+        if (s != SPT_OK) return s;
+        if (gs.d_scratch) (void)hipFree(gs.d_scratch);
+        gs.d_scratch = nullptr;
+        gs.scratch_bytes = 0;
+        gs.scratch_rows = 0;
+        s = check_hip(ctx, hipMalloc(&gs.d_scratch, need), "guide scratch");
+        if (s != SPT_OK) return s;
+        s = check_hip(ctx, hipMemsetAsync(gs.d_scratch, 0, need, ctx->stream), "guide scratch clear");
+        if (s != SPT_OK) return s;
+        gs.scratch_bytes = need;
+        gs.scratch_rows = rows;
+    }
+    const uint32_t R = gs.scratch_rows;
+    char* p = gs.d_scratch;
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += (bytes + 255) & ~size_t(255);
+        return q;
+    };
+    ts.xT = reinterpret_cast<unsigned short*>(take((size_t)R * kEnc * 2));
+    ts.h1T = reinterpret_cast<unsigned short*>(take((size_t)R * kHid * 2));
+    ts.h2T = reinterpret_cast<unsigned short*>(take((size_t)R * kHid * 2));
+    ts.h3T = reinterpret_cast<unsigned short*>(take((size_t)R * kHid * 2));
+    ts.dyT = reinterpret_cast<float*>(take((size_t)R * kOutP * 4));
+    ts.dz3T = reinterpret_cast<unsigned short*>(take((size_t)R * kHid * 2));
+    ts.dz2T = reinterpret_cast<unsigned short*>(take((size_t)R * kHid * 2));
+    ts.dz1T = reinterpret_cast<unsigned short*>(take((size_t)R * kHid * 2));
+    ts.partial = reinterpret_cast<float*>(take((size_t)splits * kParams * 4));
+    ts.rows = R;
+    if ((size_t)(p - gs.d_scratch) > gs.scratch_bytes) return set_error(ctx, SPT_ERR_NOMEM, "guide scratch layout");
+    return SPT_OK;
+}
+
+// Queues the gradient of one step (ctl->wsum and ctl->loss, grad = scaled
+// gradient) on the context stream.
+spt_status queue_gradient(spt_ctx* ctx, const float* d_rec, uint32_t n, hipEvent_t e0, hipEvent_t e1) {
+    GuideState& gs = ctx->guide;
+    hipStream_t st = ctx->stream;
+    GuideCtl* ctl = gs.ctl();
+    const uint32_t n16 = (n + 15) & ~15u;
+    const uint32_t span = std::max<uint32_t>(1024, (((n16 + 511) / 512) + 15) & ~15u);
+    const uint32_t splits = (n16 + span - 1) / span;
+    TrainScratch ts;
+    spt_status s = train_scratch(ctx, n16, splits, ts);
+    if (s != SPT_OK) return s;
+    hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, &ctl->wsum);
+    if (e0) (void)hipEventRecord(e0, st);
+    hipLaunchKernelGGL(k_train_fwd, dim3((n16 + kBlock - 1) / kBlock), dim3(kBlock), fwd_lds_bytes(), st, gs.d_theta,
+                       gs.d_wbf, d_rec, n, n16, ctl, guide_box(ctx), ts);
+    hipLaunchKernelGGL(k_train_bwd, dim3(std::min<uint32_t>((n16 + 63) / 64, (uint32_t)ctx->num_cus * 4)), dim3(kBlock),
+                       sizeof(SharedWeights), st, gs.d_theta, gs.d_wbf, n16, ts);
+    hipLaunchKernelGGL(k_train_wgrad, dim3(4 * splits), dim3(kBlock), 0, st, ts, n16, span);
+    hipLaunchKernelGGL(k_train_reduce, dim3((kParams + kBlock - 1) / kBlock), dim3(kBlock), 0, st, ts.partial, splits,
+                       gs.d_grad);
+    if (e1) (void)hipEventRecord(e1, st);
+    return check_hip(ctx, hipGetLastError(), "guide gradient launch");
 }
 
 }  // namespace
@@ -243,12 +498,10 @@ spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, floa
     hipStream_t st = ctx->stream;
     GuideCtl* ctl = gs.ctl();
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, st, ctl);
-    hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, &ctl->wsum);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (gs.event_pair(e0, e1)) (void)hipEventRecord(e0, st);
-    hipLaunchKernelGGL(k_guide_train, dim3((uint32_t)ctx->num_cus), dim3(kBlock), train_lds_bytes(), st, gs.d_theta,
-                       gs.d_wbf, d_rec, n, gs.d_grad, ctl, guide_box(ctx));
-    if (e1) (void)hipEventRecord(e1, st);
+    if (!gs.event_pair(e0, e1)) e0 = e1 = nullptr;
+    spt_status qs = queue_gradient(ctx, d_rec, n, e0, e1);
+    if (qs != SPT_OK) return qs;
     hipLaunchKernelGGL(k_guide_check, dim3(64), dim3(kBlock), 0, st, gs.d_grad, ctl);
     hipLaunchKernelGGL(k_guide_adam, dim3((kParams + kBlock - 1) / kBlock), dim3(kBlock), 0, st, gs.d_theta, gs.d_m,
                        gs.d_v, gs.d_grad, gs.d_wbf, ctl, gs.lr);
@@ -270,11 +523,8 @@ spt_status guide_grad_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, float
     hipStream_t st = ctx->stream;
     GuideCtl* ctl = gs.ctl();
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, st, ctl);
-    spt_status s = check_hip(ctx, hipMemsetAsync(gs.d_grad, 0, kParams * sizeof(float), st), "guide grad");
+    spt_status s = queue_gradient(ctx, d_rec, n, nullptr, nullptr);
     if (s != SPT_OK) return s;
-    hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, &ctl->wsum);
-    hipLaunchKernelGGL(k_guide_train, dim3((uint32_t)ctx->num_cus), dim3(kBlock), train_lds_bytes(), st, gs.d_theta,
-                       gs.d_wbf, d_rec, n, gs.d_grad, ctl, guide_box(ctx));
     s = check_hip(ctx, hipMemcpyAsync(grad_out, gs.d_grad, kParams * sizeof(float), hipMemcpyDeviceToHost, st),
                   "guide grad read");
     GuideCtl h;
