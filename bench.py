@@ -377,6 +377,14 @@ def main():
     achieved = bytes_per_ray * rays / (ext_ms * 1e-3) / 1e9
     img = film.download(np.float32, (cam.height, cam.width, 3)) / (args.steps * sps)
 
+    # HBM bytes per extension launch of this workload, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_pt_mesh_in_box.json")
+    if os.path.exists(tf) and args.rings == 512 and args.segments == 1024 and sps == 16:
+        with open(tf) as fh:
+            traffic = json.load(fh)["hbm_bytes_per_launch"]
+
     extra = {}
     if not args.no_closest_hit:
         mesh = scenes.displaced_mesh(args.rings, args.segments)
@@ -416,7 +424,7 @@ def main():
                    "baseline_note": "vs_baseline divides by 0.39 Mpaths/s, the reference README's Buddha run "
                                     "(1M tris, 2-layer BSDF, 800x600, i5 CPU)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_extend (closest-hit traversal of extension rays)",
                      "bytes_per_ray": round(bytes_per_ray, 1), "node_fetches_per_ray": round(per_ray_nodes, 2),
                      "tri_fetches_per_ray": round(per_ray_tris, 2),
