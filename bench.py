@@ -287,7 +287,7 @@ def main():
     ap.add_argument("--guide-lr", type=float, default=5e-3)
     ap.add_argument("--guide-warmup", type=int, default=16, help="1-spp training passes before the counted image")
     ap.add_argument("--guide-pass-spp", type=int, default=16, help="spp of each counted guided pass (a square)")
-    ap.add_argument("--guide-steps", type=int, default=8, help="Adam steps per training pass")
+    ap.add_argument("--guide-steps", type=int, default=4, help="Adam steps per training pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
