@@ -412,7 +412,9 @@ __device__ inline float layer_density(const spt_layer& l, V3 wo, V3 wi) {
 
 // Shading of one bounce (integrators/pathtracer.cpp:21-54) for every live
 // path of A, in material order when sorted; survivors are appended to B.
-template <bool MULTI>
+// GUIDED instantiates the guide's sampling and record capture; the plain
+// instantiation carries none of their registers.
+template <bool MULTI, bool GUIDED>
 __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathBuf B, Work w,
                                                   const uint32_t* __restrict__ cntA, uint32_t* __restrict__ cntB) {
     const uint32_t n = *cntA;
@@ -432,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
                 rad.y += thr.y * jb.bg[1];
                 rad.z += thr.z * jb.bg[2];
             }
-            if (a.train) finish_records(a, A.grec[i], rad);
+            if (GUIDED && a.train) finish_records(a, A.grec[i], rad);
             finish_path<MULTI>(a, jb, meta.w, meta.x, A.local[i], rad);
             continue;
         }
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         Draws r = make_draws(jb, key, meta.y);
         V3 bsdf, refl, wi;
         float pdf;
-        const bool guided = a.guided && a.guidable[mid];
+        const bool guided = GUIDED && a.guidable[mid];
         if (guided) {
             // one-sample mixture of the layer's sampler and the guide
             using namespace guide;
@@ -497,15 +499,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         const uint32_t nd = depth + 1;
         if (alive && (jb.max_depth >= 0) && ((int32_t)nd >= jb.max_depth)) alive = false;  // pathtracer.cpp:18
         if (!alive) {
-            if (a.train) finish_records(a, A.grec[i], rad);
+            if (GUIDED && a.train) finish_records(a, A.grec[i], rad);
             finish_path<MULTI>(a, jb, meta.w, meta.x, A.local[i], rad);
             continue;
         }
         // pathtracer.cpp:52-54, forwards: throughput *= f * cos / pdf
         const V3 wgt = (bsdf * wi.y) / pdf;
-        uint4 gr = a.train ? A.grec[i] : make_uint4(0u, 0u, 0u, 0u);
-        const bool cand = a.train && guided && gr.w < 3u && wi.y > 0.0f && bsdf.x + bsdf.y + bsdf.z > 0.0f;
-        if (a.train) {  // candidates counted per wave (one atomic)
+        uint4 gr = (GUIDED && a.train) ? A.grec[i] : make_uint4(0u, 0u, 0u, 0u);
+        const bool cand = GUIDED && a.train && guided && gr.w < 3u && wi.y > 0.0f && bsdf.x + bsdf.y + bsdf.z > 0.0f;
+        if (GUIDED && a.train) {  // candidates counted per wave (one atomic)
             const uint64_t cb = __ballot(cand);
             if (cb && __lane_id() == (unsigned)(__ffsll((unsigned long long)cb) - 1))
                 atomicAdd(a.rec_count + 1, (uint32_t)__popcll(cb));
@@ -534,7 +536,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         B.meta[slot] = make_uint4(meta.x, r.k, nd, meta.w);
         B.key[slot] = key;
         B.local[slot] = A.local[i];
-        if (a.train) B.grec[slot] = gr;
+        if (GUIDED && a.train) B.grec[slot] = gr;
     }
 }
 
@@ -768,7 +770,10 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
                                st, a, A, w, cntA, ctx->guide.d_theta, ctx->guide.d_wbf, guide_box(ctx));
             if (timed && !gev.empty()) (void)hipEventRecord(gev.back(), st);
         }
-        hipLaunchKernelGGL((k_shade<MULTI>), dim3(shade_grid), dim3(kBlock), 0, st, a, A, B, w, cntA, cntB);
+        if (a.guided)
+            hipLaunchKernelGGL((k_shade<MULTI, true>), dim3(shade_grid), dim3(kBlock), 0, st, a, A, B, w, cntA, cntB);
+        else
+            hipLaunchKernelGGL((k_shade<MULTI, false>), dim3(shade_grid), dim3(kBlock), 0, st, a, A, B, w, cntA, cntB);
         hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, st, cntA, w.cnt + 8);
         return check_hip(ctx, hipGetLastError(), "bounce launch");
     };
