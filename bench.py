@@ -134,7 +134,7 @@ def guide_experiment(args):
         p.spp, p.seed = osp, 0x5EED0F0A
         p.spp_begin, p.spp_count = b, min(chunk, osp - b)
         p.flags = L.SPT_RENDER_FILM64
-        dev.render(cam, p, None, film64.ptr, None)
+        dev.render(cam, p, None, film64, None)
     dev.sync()
     ref = film64.download(np.float64, (H, W, 3)) / osp
     log(f"reference {osp} spp: {time.perf_counter() - t0:.1f} s, mean {ref.reshape(-1, 3).mean(0)}")
@@ -148,7 +148,7 @@ def guide_experiment(args):
         p.flags = L.SPT_RENDER_FILM32
         dev.sync()
         t = time.perf_counter()
-        dev.render(cam, p, film.ptr, None, None)
+        dev.render(cam, p, film, None, None)
         dev.sync()
         return time.perf_counter() - t, film.download(np.float32, (H, W, 3)) / spp
 
@@ -161,7 +161,7 @@ def guide_experiment(args):
     # images discarded but their time counted), then passes of 4 spp that
     # keep training; the running image of the post-warm-up passes is compared
     # with the reference after each pass (comparison time not counted)
-    dev.guide_init(seed=1, lr=args.guide_lr, batch=262144, steps_per_pass=args.guide_steps)
+    dev.guide_init(seed=1, lr=args.guide_lr, batch=262144, steps_per_pass=args.guide_steps, flags=args.guide_flags)
     t_guided, t_train, hit = 0.0, 0.0, None
     curve = []
     k = 0
@@ -174,7 +174,7 @@ def guide_experiment(args):
         p.flags = L.SPT_RENDER_FILM32 | L.SPT_RENDER_GUIDED | L.SPT_RENDER_GUIDE_TRAIN
         dev.sync()
         t = time.perf_counter()
-        dev.render(cam, p, tgt.ptr, None, None)
+        dev.render(cam, p, tgt, None, None)
         t1 = time.perf_counter()
         dev.guide_train_collected(wait=False)
         dev.sync()
@@ -288,6 +288,8 @@ def main():
     ap.add_argument("--guide-warmup", type=int, default=16, help="1-spp training passes before the counted image")
     ap.add_argument("--guide-pass-spp", type=int, default=16, help="spp of each counted guided pass (a square)")
     ap.add_argument("--guide-steps", type=int, default=4, help="Adam steps per training pass")
+    ap.add_argument("--guide-flags", type=int, default=0,
+                    help="SPT_GUIDE_* options: 1 guide layers under a smooth coat, 2 radiance x BSDF-lobe product")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -342,7 +344,7 @@ def main():
         p.spp_begin = (i * sps) % spp_total
         p.spp_count = sps
         p.flags = flags | (L.SPT_RENDER_NO_SORT if args.no_sort else 0)
-        dev.render(cam, p, film.ptr, None, None)
+        dev.render(cam, p, film, None, None)
 
     for i in range(args.warmup):
         step(i)

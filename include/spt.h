@@ -417,7 +417,17 @@ typedef struct spt_guide_config {
     uint32_t steps_per_pass; /* a training render records up to batch x steps_per_pass
                                 records; spt_guide_train_collected takes one Adam step
                                 per batch of them (default 1) */
+    uint32_t flags;        /* SPT_GUIDE_* options (default 0) */
 } spt_guide_config;
+
+/* Guide options. COATED: also guide a Lambertian / GGX layer under one smooth
+ * dielectric coat (the first visit of the base layer in the layered walk
+ * draws from the mixture, the guide's exit direction mapped under the coat).
+ * PRODUCT: learn incident radiance (training weight L_i / p) and propose
+ * directions from its product with a vMF stand-in of the BSDF lobe, instead
+ * of learning the product f L_i cos directly. */
+#define SPT_GUIDE_COATED (1u << 0)
+#define SPT_GUIDE_PRODUCT (1u << 1)
 
 typedef struct spt_guide_stats {
     uint64_t steps;

@@ -44,6 +44,8 @@ SPT_RENDER_GUIDED = 1 << 4
 SPT_RENDER_FETCH_STATS = 1 << 5
 SPT_RENDER_TIME_KERNELS = 1 << 6
 SPT_RENDER_GUIDE_TRAIN = 1 << 7
+SPT_GUIDE_COATED = 1 << 0
+SPT_GUIDE_PRODUCT = 1 << 1
 SPT_GUIDE_PARAMS = 13552
 SPT_GUIDE_RECORD_FLOATS = 24
 SPT_TRACE_CLOSEST = 0
@@ -96,7 +98,8 @@ class RenderStats(C.Structure):
 
 class GuideConfig(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("learning_rate", C.c_float), ("loss_scale", C.c_float),
-                ("bsdf_fraction", C.c_float), ("batch", C.c_uint32), ("steps_per_pass", C.c_uint32)]
+                ("bsdf_fraction", C.c_float), ("batch", C.c_uint32), ("steps_per_pass", C.c_uint32),
+                ("flags", C.c_uint32)]
 
 
 class GuideStats(C.Structure):

@@ -339,8 +339,34 @@ class Device:
         return out
 
     # path tracing ------------------------------------------------------
+    @staticmethod
+    def call_paths(cam: L.Camera, params: L.RenderParams) -> int:
+        """Paths of one spt_render call: viewport pixels x sample count, where
+        the regular and jittered samplers round spp up to n^2, n = ceil(sqrt(spp))
+        (jittered_pixel_sampler.cpp:6-8)."""
+        spp = params.spp
+        if params.pixel_sampler != L.SPT_PIXEL_UNIFORM:
+            spp = int(np.ceil(np.sqrt(spp))) ** 2
+        cnt = params.spp_count or max(spp - params.spp_begin, 0)
+        vw, vh = params.viewport[2], params.viewport[3]
+        if vw == 0:
+            vw, vh = cam.width, cam.height
+        return vw * vh * cnt
+
     def render(self, cam: L.Camera, params: L.RenderParams, d_rgb32=None, d_rgb64=None, d_path=None):
-        self._check(self.lib.spt_render(self.h, C.byref(cam), C.byref(params), d_rgb32, d_rgb64, d_path))
+        """spt_render. Outputs are DeviceArrays (sizes checked here, since the
+        C ABI takes bare pointers) or raw device addresses (unchecked)."""
+        npix = cam.width * cam.height
+        need = ((d_rgb32, npix * 12, "FP32 film"), (d_rgb64, npix * 24, "FP64 film"),
+                (d_path, self.call_paths(cam, params) * 12, "per-path output"))
+        ptrs = []
+        for buf, nbytes, what in need:
+            if isinstance(buf, DeviceArray):
+                if buf.nbytes < nbytes:
+                    raise ValueError(f"{what} needs {nbytes} bytes, buffer has {buf.nbytes}")
+                buf = buf.ptr
+            ptrs.append(buf)
+        self._check(self.lib.spt_render(self.h, C.byref(cam), C.byref(params), *ptrs))
 
     # tuning ------------------------------------------------------------
     def set_tuning(self, **kw):
@@ -362,9 +388,9 @@ class Device:
 
     # learned guide -----------------------------------------------------
     def guide_init(self, seed: int = 1, lr: float = 5e-3, loss_scale: float = 65536.0, bsdf_fraction: float = 0.5,
-                   batch: int = 262144, steps_per_pass: int = 1):
+                   batch: int = 262144, steps_per_pass: int = 1, flags: int = 0):
         cfg = L.GuideConfig(seed=seed, learning_rate=lr, loss_scale=loss_scale, bsdf_fraction=bsdf_fraction,
-                            batch=batch, steps_per_pass=steps_per_pass)
+                            batch=batch, steps_per_pass=steps_per_pass, flags=flags)
         self._check(self.lib.spt_guide_init(self.h, C.byref(cfg)))
 
     def guide_set_params(self, theta: np.ndarray):
@@ -460,8 +486,7 @@ class Device:
             p.rng = L.SPT_RNG_STREAM
             p.stream_stride = stream.shape[1]
             p.d_stream = dstream.ptr
-        self.render(cam, p, None if film64 else film.ptr, film.ptr if film64 else None,
-                    dpath.ptr if dpath else None)
+        self.render(cam, p, None if film64 else film, film if film64 else None, dpath)
         self.sync()
         img = film.download(np.float64 if film64 else np.float32, (cam.height, cam.width, 3))
         img = img / (np.float64(cnt) if film64 else np.float32(cnt))

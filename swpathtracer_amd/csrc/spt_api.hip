@@ -455,14 +455,19 @@ spt_status spt_set_materials(spt_ctx* ctx, const spt_material* mats, uint32_t n_
     if (s != SPT_OK) return s;
     ctx->n_mats = n_mats;
     ctx->h_mats.assign(mats, mats + n_mats);
-    // vertices the learned guide may sample: one layer with an evaluable density
+    // vertices the learned guide may sample: a layer with an evaluable
+    // density, alone or under one smooth dielectric coat
+    auto evaluable = [](const spt_layer& l) {
+        return l.type == SPT_BSDF_LAMBERTIAN || (l.type == SPT_BSDF_GGX_CONDUCTOR && l.sampler == SPT_SAMPLER_IMPORTANCE);
+    };
     std::vector<uint8_t> g(n_mats, 0);
     for (uint32_t i = 0; i < n_mats; ++i) {
-        const spt_layer& l = mats[i].layers[0];
-        g[i] = (mats[i].n_layers == 1 &&
-                (l.type == SPT_BSDF_LAMBERTIAN || (l.type == SPT_BSDF_GGX_CONDUCTOR && l.sampler == SPT_SAMPLER_IMPORTANCE)))
-                   ? 1
-                   : 0;
+        const spt_material& m = mats[i];
+        if (m.n_layers == 1 && evaluable(m.layers[0]))
+            g[i] = kGuideSingle;
+        else if (m.n_layers == 2 && m.layers[0].type == SPT_BSDF_SMOOTH_DIELECTRIC && m.layers[0].ior >= 1.0f &&
+                 evaluable(m.layers[1]))
+            g[i] = kGuideCoated;
     }
     if (ctx->d_guidable) (void)hipFree(ctx->d_guidable);
     ctx->d_guidable = nullptr;

@@ -28,6 +28,11 @@ struct Node4 {
 static_assert(sizeof(Node4) == 128, "Node4 layout");
 constexpr uint32_t kNoChild = 0xFFFFFFFFu;
 
+// Materials the learned guide samples: one Lambertian / importance-sampled
+// GGX layer (kGuideSingle), or such a layer under one smooth dielectric coat
+// of IOR >= 1 (kGuideCoated: the guide drives the base layer's directions).
+constexpr uint8_t kGuideSingle = 1, kGuideCoated = 2;
+
 // Scene as the kernels see it.
 struct DevScene {
     const float4* nodes4;   // n_nodes4 x 8 float4 (Node4), persistent traversal
@@ -537,6 +542,7 @@ struct GuideState {
     uint32_t records_alloc = 0;  // allocated record slots
     uint64_t collected = 0;
     float lr = 5e-3f, loss_scale = 65536.0f, bsdf_fraction = 0.5f;  // loss_scale: the configured start
+    uint32_t flags = 0;  // SPT_GUIDE_*
     uint32_t batch = 262144;
     double train_ms = 0.0;
     float* d_gout = nullptr;  // per-path raw outputs during a guided render
@@ -607,7 +613,7 @@ struct spt_ctx {
     uint32_t max_mat = 0;  // largest material id referenced by a triangle
     bool has_scene = false;
     std::vector<spt_material> h_mats;
-    uint8_t* d_guidable = nullptr;  // per material: 1 = single Lambertian / GGX layer (guide-sampled)
+    uint8_t* d_guidable = nullptr;  // per material: kGuideSingle / kGuideCoated, 0 = not guide-sampled
     // traversal stack spill buffer
     uint32_t* d_spill = nullptr;
     size_t spill_words = 0;

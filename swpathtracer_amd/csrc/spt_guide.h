@@ -248,6 +248,33 @@ __device__ inline float log_vmf(const Lobe& L, V3 d) {
     return logf(L.kappa) - 1.8378770664093453f - logf(-expm1f(-2.0f * L.kappa)) + L.kappa * (dot(L.mu, d) - 1.0f);
 }
 
+// log of the vMF normalisation kappa / (2 pi (1 - e^{-2 kappa}))
+__device__ inline float log_vmf_norm(float k) { return logf(k) - 1.8378770664093453f - logf(-expm1f(-2.0f * k)); }
+
+// Product of the mixture with one vMF lobe (mb, kb), renormalised (product
+// importance sampling with vMF lobes, Herholz et al. 2016):
+// vMF(mu_i, k_i) vMF(mb, kb) = C_i C_b / C' e^{k' - k_i - kb} vMF(mu', k')
+// with k' mu' = k_i mu_i + kb mb. The result is again a mixture, so sampling
+// and density stay exact.
+__device__ inline void mixture_product(Lobe L[kLobes], V3 mb, float kb) {
+    const float lcb = log_vmf_norm(kb);
+    float lw[kLobes];
+    float m = -3.0e38f;
+    for (int i = 0; i < kLobes; ++i) {
+        const V3 v = L[i].mu * L[i].kappa + mb * kb;
+        const float nv = sqrtf(dot(v, v));
+        const float kp = fmaxf(nv, kKappaMin);
+        lw[i] = L[i].logw + log_vmf_norm(L[i].kappa) + lcb - log_vmf_norm(kp) + (nv - L[i].kappa - kb);
+        L[i].mu = (nv > 1e-6f) ? v * (1.0f / nv) : mb;
+        L[i].kappa = kp;
+        m = fmaxf(m, lw[i]);
+    }
+    float sum = 0.0f;
+    for (int i = 0; i < kLobes; ++i) sum += expf(lw[i] - m);
+    const float lse = m + logf(sum);
+    for (int i = 0; i < kLobes; ++i) L[i].logw = lw[i] - lse;
+}
+
 __device__ inline float mixture_pdf(const Lobe L[kLobes], V3 d) {
     float q = 0.0f;
     for (int i = 0; i < kLobes; ++i) q += expf(L[i].logw + log_vmf(L[i], d));

@@ -581,7 +581,8 @@ spt_status spt_guide_init(spt_ctx* ctx, const spt_guide_config* cfg) {
     if (cfg) c = *cfg;
     if (c.steps_per_pass == 0) c.steps_per_pass = 1;
     if (!(c.bsdf_fraction > 0.0f && c.bsdf_fraction < 1.0f) || c.batch == 0 || !(c.learning_rate > 0.0f) ||
-        !(c.loss_scale > 0.0f) || c.steps_per_pass > 64 || (uint64_t)c.batch * c.steps_per_pass >= (1ull << 31))
+        !(c.loss_scale > 0.0f) || c.steps_per_pass > 64 || (uint64_t)c.batch * c.steps_per_pass >= (1ull << 31) ||
+        (c.flags & ~(SPT_GUIDE_COATED | SPT_GUIDE_PRODUCT)))
         return set_error(ctx, SPT_ERR_INVALID, "guide config out of range");
     const uint32_t cap = c.batch * c.steps_per_pass;
     GuideState& gs = ctx->guide;
@@ -611,6 +612,7 @@ spt_status spt_guide_init(spt_ctx* ctx, const spt_guide_config* cfg) {
     gs.lr = c.learning_rate;
     gs.loss_scale = c.loss_scale;
     gs.bsdf_fraction = c.bsdf_fraction;
+    gs.flags = c.flags;
     gs.batch = c.batch;
     gs.train_ms = 0.0;
     gs.kernel_ms = 0.0;

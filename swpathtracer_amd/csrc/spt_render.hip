@@ -62,7 +62,8 @@ struct RenderArgs {
     uint32_t keys;  // sort keys (n_mats + 1), 0 = no sort
     // learned guide
     uint32_t guided, train;
-    const uint8_t* guidable;  // per material: single Lambertian / GGX layer
+    uint32_t gflags;          // SPT_GUIDE_*
+    const uint8_t* guidable;  // per material: kGuideSingle / kGuideCoated / 0
     float bsdf_frac;
     float* gout;              // per path slot: 40 raw network outputs
     float* records;           // training records (guide::kRecord floats each)
@@ -83,6 +84,12 @@ __device__ inline void finish_records(const RenderArgs& a, uint4 gr, float4 L) {
         const float li = lum(V3{L.x - r[14], L.y - r[15], L.z - r[16]});
         r[12] = (lt > 0.0f && li > 0.0f) ? r[13] * li / lt : 0.0f;
     }
+}
+
+// Guide mode of a material in this call: coated layers only with SPT_GUIDE_COATED.
+__device__ inline uint32_t guide_mode(const RenderArgs& a, uint32_t mid) {
+    const uint32_t g = a.guidable[mid];
+    return (g == kGuideCoated && !(a.gflags & SPT_GUIDE_COATED)) ? 0u : g;
 }
 
 template <bool MULTI>
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_guide_paths(RenderArgs a, PathBuf A,
             slot = a.keys ? w.perm[j] : j;
             const float4 hv = w.hit[slot];
             const uint32_t idx = __float_as_uint(hv.y);
-            mine = (idx != kNoHit) && a.guidable[w.skey[slot]];
+            mine = (idx != kNoHit) && guide_mode(a, w.skey[slot]) != 0u;
             if (mine) {
                 const float4 o4 = A.o[slot], d4 = A.d[slot];
                 const V3 d{d4.x, d4.y, d4.z};
@@ -408,6 +415,128 @@ __device__ inline float layer_density(const spt_layer& l, V3 wo, V3 wi) {
         return (l.sampler == SPT_SAMPLER_UNIFORM) ? 1.0f / (2.0f * kPi) : wi.y / kPi;
     }
     return ggx_pdf(l, wo, wi);
+}
+
+// Guide density of a direction w2 inside a smooth dielectric coat of IOR eta
+// (local frame, pointing up): the guide proposes the exit direction above the
+// coat, and w2 is the direction that refracts to it (tangential part scaled
+// by 1/eta). p(w2) = p(exit) |d exit / d w2| = p(exit) eta^2 cos(w2) /
+// cos(exit). Directions past the critical angle are never proposed.
+__device__ inline float coat_guide_density(const guide::Lobe* lobes, const Frame& fr, V3 w2, float eta) {
+    const float t2 = eta * eta * (w2.x * w2.x + w2.z * w2.z);
+    if (w2.y <= 0.0f || t2 >= 1.0f) return 0.0f;
+    const float cy = sqrtf(1.0f - t2);
+    const V3 out{eta * w2.x, cy, eta * w2.z};
+    return guide::mixture_pdf(lobes, to_world(fr, out)) * (eta * eta * w2.y / cy);
+}
+
+// vMF stand-in (local frame) for the lobe of a guided layer, multiplied with
+// the learned incident radiance to propose directions: Lambertian, the
+// normal with kappa 3 (mean cosine 2/3, that of the cosine lobe); GGX, the
+// mirror direction with kappa 1 / (4 alpha^2) (reflected spread about twice
+// the microfacet spread alpha), widened by 'widen' (the IOR of a smooth coat
+// above the layer, which scales exit angles).
+__device__ inline void bsdf_lobe(const spt_layer& l, V3 wo, float widen, V3& mb, float& kb) {
+    if (l.type == SPT_BSDF_LAMBERTIAN) {
+        mb = V3{0.0f, 1.0f, 0.0f};
+        kb = 3.0f;
+        return;
+    }
+    mb = V3{-wo.x, wo.y, -wo.z};
+    const float s = l.roughness * widen;
+    kb = fminf(fmaxf(0.25f / fmaxf(s * s, 1e-8f), guide::kKappaMin), 1e3f);
+}
+
+// The layered random walk of layered_eval (materials/layered_bsdf.cpp:10-74)
+// for a smooth dielectric coat over one opaque layer with an evaluable
+// density, where the first visit of the opaque layer draws its direction from
+// a one-sample mixture: the layer's own sampler with probability c, else the
+// guide's exit direction mapped under the coat. The step density is the
+// mixture's, so the walk weight estimates the same BSDF without bias. Later
+// visits (after total internal reflection under the coat, where the guide
+// has no density) use the layer's own sampler: mixing there would double the
+// weight of every such revisit. 'via' is set when the walk reached the
+// opaque layer.
+__device__ inline void layered_eval_guided(const spt_material& m, V3 wo, const Frame& frm, const guide::Lobe* lobes,
+                                           float c, V3& bsdf, V3& refl, float& pdf, V3& wi, Draws& r, bool& via) {
+    const spt_layer& coat = m.layers[0];
+    const spt_layer& base = m.layers[1];
+    const float eta = coat.ior, ieta = 1.0f / coat.ior;
+    V3 fb{1.0f, 1.0f, 1.0f}, fr{1.0f, 1.0f, 1.0f};
+    float fp = 1.0f;
+    V3 nwo = wo;
+    bool reached_inf = false;
+    int bounce = 0;
+    int id = (nwo.y > 0.0f) ? 0 : 1;
+    const float inf = __builtin_inff();
+    via = false;
+    while ((id > -1) && (id < 2) && (fp > 0.0f) && (!reached_inf) && (bounce < 500)) {
+        if (id == 0) {  // coat: the layer's own sampler, as in layered_eval
+            const BSample s = layer_sample(coat, nwo, r);
+            wi = s.wi;
+            fp *= s.pdf;
+            if (!(fp < inf)) reached_inf = true;
+            if (wi.y > 0.0f) {
+                fb = fb * layer_f(coat, nwo, wi);
+                fr = fr * layer_reflectance(coat, nwo, wi);
+                --id;
+            } else if (wi.y < 0.0f) {
+                fb = fb * (layer_f(coat, nwo, wi) * wi.y);
+                fr = fr * layer_reflectance(coat, nwo, wi);
+                ++id;
+            } else {
+                fp = 0.0f;
+            }
+        } else if (via) {  // opaque base, revisited: its own sampler, as in layered_eval
+            const BSample s = layer_sample(base, nwo, r);
+            wi = s.wi;
+            fp *= s.pdf;
+            if (!(fp < inf)) reached_inf = true;
+            if (wi.y > 0.0f) {
+                fb = fb * (layer_f(base, nwo, wi) * wi.y);
+                fr = fr * layer_reflectance(base, nwo, wi);
+                --id;
+            } else {
+                fp = 0.0f;
+            }
+        } else {  // opaque base, first visit: one-sample mixture with the guide
+            via = true;
+            if (r.next() < c) {
+                wi = layer_sample(base, nwo, r).wi;
+            } else {
+                const float u0 = r.next(), u1 = r.next(), u2 = r.next();
+                const V3 g = to_local(frm, guide::mixture_sample(lobes, u0, u1, u2));
+                if (g.y > 0.0f) {
+                    const float tx = g.x * ieta, tz = g.z * ieta;
+                    wi = V3{tx, sqrtf(gmax(0.0f, 1.0f - tx * tx - tz * tz)), tz};
+                } else {
+                    wi = V3{0.0f, -1.0f, 0.0f};  // exit below the surface: no contribution
+                }
+            }
+            const float p = c * layer_density(base, nwo, wi) + (1.0f - c) * coat_guide_density(lobes, frm, wi, eta);
+            if ((wi.y > 0.0f) && (p > 0.0f)) {
+                fp *= p;
+                if (!(fp < inf)) reached_inf = true;
+                fb = fb * (layer_f(base, nwo, wi) * wi.y);
+                fr = fr * layer_reflectance(base, nwo, wi);
+                --id;
+            } else {
+                fp = 0.0f;
+            }
+        }
+        if (!((fb.x < inf) && (fb.y < inf) && (fb.z < inf))) reached_inf = true;
+        nwo = -wi;
+        ++bounce;
+    }
+    if ((bounce < 500) && !reached_inf && (fp > 0.0f)) {
+        pdf = fp;
+        bsdf = fb;
+        refl = fr;
+    } else {
+        pdf = 1.0f;
+        bsdf = V3{0.0f, 0.0f, 0.0f};
+        refl = V3{0.0f, 0.0f, 0.0f};
+    }
 }
 
 // Shading of one bounce (integrators/pathtracer.cpp:21-54) for every live
@@ -455,13 +584,43 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         Draws r = make_draws(jb, key, meta.y);
         V3 bsdf, refl, wi;
         float pdf;
-        const bool guided = GUIDED && a.guidable[mid];
-        if (guided) {
-            // one-sample mixture of the layer's sampler and the guide
+        const uint32_t gmode = GUIDED ? guide_mode(a, mid) : 0u;
+        const bool prod = GUIDED && (a.gflags & SPT_GUIDE_PRODUCT);
+        const bool guided = gmode != 0u;
+        bool via = true;  // the sampled direction came from a guided step
+        float pmarg = 0.0f;  // PRODUCT: density of wi, the training weight's 1/p
+        if (gmode == kGuideCoated) {
+            // with PRODUCT: the learned incident radiance times the coated
+            // lobe in exit space
+            using namespace guide;
+            Lobe lobes[kLobes];
+            mixture_params(a.gout + (size_t)i * kOut, lobes);
+            V3 mb;
+            float kb;
+            bsdf_lobe(m.layers[1], wo, m.layers[0].ior, mb, kb);
+            const V3 mbw = to_world(fr, mb);
+            if (prod) mixture_product(lobes, mbw, kb);
+            layered_eval_guided(m, wo, fr, lobes, a.bsdf_frac, bsdf, refl, pdf, wi, r, via);
+            if (prod && a.train && via && wi.y > 0.0f) {  // exit density, the lobe standing in for the walk
+                const V3 e = to_world(fr, wi);
+                Lobe bl;
+                bl.mu = mbw;
+                bl.kappa = kb;
+                pmarg = a.bsdf_frac * expf(log_vmf(bl, e)) + (1.0f - a.bsdf_frac) * mixture_pdf(lobes, e);
+            }
+        } else if (guided) {
+            // one-sample mixture of the layer's sampler and the learned
+            // incident radiance times a vMF stand-in of the layer's lobe
             using namespace guide;
             const spt_layer& l = m.layers[0];
             Lobe lobes[kLobes];
             mixture_params(a.gout + (size_t)i * kOut, lobes);
+            if (prod) {
+                V3 mb;
+                float kb;
+                bsdf_lobe(l, wo, 1.0f, mb, kb);
+                mixture_product(lobes, to_world(fr, mb), kb);
+            }
             if (r.next() < a.bsdf_frac) {
                 wi = layer_sample(l, wo, r).wi;
             } else {
@@ -470,6 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
             }
             pdf = a.bsdf_frac * layer_density(l, wo, wi) +
                   (1.0f - a.bsdf_frac) * mixture_pdf(lobes, to_world(fr, wi));
+            pmarg = pdf;
             if (wi.y > 0.0f && pdf > 0.0f) {
                 bsdf = layer_f(l, wo, wi);
                 refl = layer_reflectance(l, wo, wi);
@@ -487,7 +647,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         rad.y += thr.y * m.emission[1];
         rad.z += thr.z * m.emission[2];
         const float kk = gmax(gmax(refl.x, refl.y), refl.z);  // pathtracer.cpp:41
-        const float rec_w = lum(bsdf) * wi.y / pdf;           // training weight, before roulette
+        // training weight before roulette: L_i / p (PRODUCT, radiance target)
+        // or f cos / p L_i (the product target)
+        const float rec_w = prod ? ((pmarg > 0.0f) ? 1.0f / pmarg : 0.0f) : lum(bsdf) * wi.y / pdf;
         bool alive = true;
         if ((jb.rr_depth >= 0) && ((int32_t)depth >= jb.rr_depth)) {  // pathtracer.cpp:43-49
             if ((r.next() < kk) && (depth < 1000u))
@@ -506,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
         // pathtracer.cpp:52-54, forwards: throughput *= f * cos / pdf
         const V3 wgt = (bsdf * wi.y) / pdf;
         uint4 gr = (GUIDED && a.train) ? A.grec[i] : make_uint4(0u, 0u, 0u, 0u);
-        const bool cand = GUIDED && a.train && guided && gr.w < 3u && wi.y > 0.0f && bsdf.x + bsdf.y + bsdf.z > 0.0f;
+        const bool cand = GUIDED && a.train && guided && via && gr.w < 3u && wi.y > 0.0f && bsdf.x + bsdf.y + bsdf.z > 0.0f;
         if (GUIDED && a.train) {  // candidates counted per wave (one atomic)
             const uint64_t cb = __ballot(cand);
             if (cb && __lane_id() == (unsigned)(__ffsll((unsigned long long)cb) - 1))
@@ -907,6 +1069,7 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     if (a.guided && !ctx->guide.ready) return set_error(ctx, SPT_ERR_STATE, "guided rendering needs spt_guide_init");
     a.guidable = ctx->d_guidable;
     a.bsdf_frac = ctx->guide.bsdf_fraction;
+    a.gflags = ctx->guide.flags;
     a.records = ctx->guide.d_records;
     a.rec_cap = ctx->guide.records_cap;
     // record about records_cap of this render's candidate vertices, estimated

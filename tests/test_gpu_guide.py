@@ -196,8 +196,9 @@ def _furnace(device, sc):
     return bvh
 
 
+@pytest.mark.parametrize("flags", [0, L.SPT_GUIDE_PRODUCT])
 @pytest.mark.parametrize("sharp", [False, True])
-def test_guided_furnace_is_unbiased(device, sharp):
+def test_guided_furnace_is_unbiased(device, sharp, flags):
     """Lambertian sphere of albedo rho in the furnace: one bounce to the
     emitter, so every path's expectation is 0.5 rho. Guided sampling mixes
     0.5 BSDF + 0.5 guide; weights are bounded by 2 rho, so the mean is within
@@ -207,7 +208,7 @@ def test_guided_furnace_is_unbiased(device, sharp):
     rho = np.array([0.999, 0.5, 0.25])
     sc = scenes.furnace(albedo=tuple(rho), width=64, height=64, spp=16)
     _furnace(device, sc)
-    device.guide_init(seed=3)
+    device.guide_init(seed=3, flags=flags)
     if sharp:
         th = G.unpack(G.init_params(3).astype(np.float64).copy())
         th["W4"][:] = 0.0
@@ -230,12 +231,51 @@ def test_guided_furnace_is_unbiased(device, sharp):
     assert x.std(0)[0] > 0  # the guide did sample
 
 
-def test_guided_training_render_collects_records(device):
+@pytest.mark.parametrize("flags", [L.SPT_GUIDE_COATED, L.SPT_GUIDE_COATED | L.SPT_GUIDE_PRODUCT])
+@pytest.mark.parametrize("sharp", [False, True])
+def test_guided_coated_walk_is_unbiased(device, sharp, flags):
+    """Layered sphere (smooth dielectric IOR 1.5 over GGX gold, alpha 0.2) in
+    the furnace: the guide drives the base layer's directions through the coat
+    (exit direction mapped under the coat, density times the refraction
+    Jacobian). The mean over all paths must match the plain walk's within a
+    few standard errors; a wrong Jacobian or density shifts it by percents.
+    'sharp' uses kappa ~ 150 lobes, partly below the surface. Without
+    SPT_GUIDE_COATED the material is not guided and the paths are the plain
+    ones."""
+    sc = scenes.furnace(width=64, height=64, spp=64)
+    sc.materials[1] = api.material([api.smooth_dielectric(1.5), api.ggx_conductor(0.2)])
+    _furnace(device, sc)
+    device.guide_init(seed=5, flags=flags)
+    if sharp:
+        th = G.unpack(G.init_params(5).astype(np.float64).copy())
+        th["W4"][:] = 0.0
+        rng = np.random.default_rng(4)
+        b4 = th["b4"]
+        for k in range(G.LOBES):
+            b4[5 * k:5 * k + 3] = _unit(rng, 1)[0]
+            b4[5 * k + 3] = np.log(150.0)
+            b4[5 * k + 4] = rng.normal()
+        device.guide_set_params(G.pack(th).astype(np.float32))
+    _, guided = device.render_image(sc.camera, sc.integrator, per_path=True, extra_flags=L.SPT_RENDER_GUIDED)
+    _, plain = device.render_image(sc.camera, sc.integrator, per_path=True)
+    g, p = guided.astype(np.float64), plain.astype(np.float64)
+    assert np.all(np.isfinite(g)) and np.all(g >= 0)
+    assert np.abs(g - p).max() > 0.01  # the guide changed paths
+    se = np.sqrt(g.var(0) / len(g) + p.var(0) / len(p))
+    print("guided", g.mean(0), "plain", p.mean(0), "se", se)
+    assert np.all(np.abs(g.mean(0) - p.mean(0)) < 5 * se)
+    device.guide_init(seed=5)
+    _, unguided = device.render_image(sc.camera, sc.integrator, per_path=True, extra_flags=L.SPT_RENDER_GUIDED)
+    np.testing.assert_array_equal(unguided, plain)
+
+
+@pytest.mark.parametrize("flags", [0, L.SPT_GUIDE_COATED | L.SPT_GUIDE_PRODUCT])
+def test_guided_training_render_collects_records(device, flags):
     sc = scenes.slit_box(rings=32, segments=64, width=48, height=36, spp=4)
     bvh = api.Bvh(sc.verts)
     device.set_scene(bvh, sc.normals, sc.mat_ids)
     device.set_materials(sc.materials)
-    device.guide_init(seed=1, batch=65536)
+    device.guide_init(seed=1, batch=65536, flags=flags)
     img, _ = device.render_image(sc.camera, sc.integrator, extra_flags=L.SPT_RENDER_GUIDED | L.SPT_RENDER_GUIDE_TRAIN)
     st = device.guide_stats()
     assert st["records"] > 1000

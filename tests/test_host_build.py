@@ -113,3 +113,21 @@ def test_invalid_inputs_are_rejected_without_a_device():
     v = np.zeros((1, 9), np.float32)
     assert lib.spt_bvh_build(v.ctypes.data, 1, 1, None) == L.SPT_ERR_INVALID
     assert lib.spt_trace(None, None, 0, None, 0, None, None) == L.SPT_ERR_INVALID
+
+
+def test_call_paths_counts_the_rounded_sample_grid():
+    """Output sizing of one render call: the regular and jittered samplers
+    render n^2 samples with n = ceil(sqrt(spp)) (jittered_pixel_sampler.cpp:6-8),
+    so a per-path buffer sized for spp paths per pixel would be too small."""
+    cam = L.Camera()
+    cam.width, cam.height = 10, 4
+    p = api.Integrator(spp=8, pixel_sampler="jittered").params()
+    assert api.Device.call_paths(cam, p) == 40 * 9
+    p = api.Integrator(spp=8, pixel_sampler="uniform").params()
+    assert api.Device.call_paths(cam, p) == 40 * 8
+    p = api.Integrator(spp=16, pixel_sampler="regular").params()
+    p.spp_begin = 5
+    assert api.Device.call_paths(cam, p) == 40 * 11
+    p.spp_count = 3
+    p.viewport = (C.c_uint32 * 4)(1, 2, 3, 2)
+    assert api.Device.call_paths(cam, p) == 6 * 3
