@@ -284,10 +284,10 @@ def main():
     ap.add_argument("--guide-full", action="store_true", help="keep rendering after equal RMSE (full curve)")
     ap.add_argument("--guide-res", type=int, nargs=2, default=(800, 600))
     ap.add_argument("--oracle-spp", type=int, default=8192, help="spp of the FP64-film reference image")
-    ap.add_argument("--guide-lr", type=float, default=5e-3)
+    ap.add_argument("--guide-lr", type=float, default=1e-2)
     ap.add_argument("--guide-warmup", type=int, default=16, help="1-spp training passes before the counted image")
     ap.add_argument("--guide-pass-spp", type=int, default=16, help="spp of each counted guided pass (a square)")
-    ap.add_argument("--guide-steps", type=int, default=4, help="Adam steps per training pass")
+    ap.add_argument("--guide-steps", type=int, default=8, help="Adam steps per training pass")
     ap.add_argument("--guide-flags", type=int, default=0,
                     help="SPT_GUIDE_* options: 1 guide layers under a smooth coat, 2 radiance x BSDF-lobe product")
     ap.add_argument("--copy-probe", action="store_true",
