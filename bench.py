@@ -91,9 +91,9 @@ def closest_hit_bench(dev, mesh, n_rays, reps=5):
     nodes = d_nodes.download(np.uint32, n_rays).astype(np.float64)
     tris = d_tris.download(np.uint32, n_rays).astype(np.float64)
     hits = d_hits.download(np.uint32, (n_rays, 2))
-    byts = 32 * n_rays + 8 * n_rays + 128 * nodes.sum() + 48 * tris.sum()
+    byts = 32 * n_rays + 8 * n_rays + 32 * nodes.sum() + 48 * tris.sum()  # tree data in 32-B units
     return {"rays": n_rays, "ms": ms, "grays_per_s": n_rays / (ms * 1e-3) / 1e9,
-            "mean_nodes": nodes.mean(), "mean_tris": tris.mean(), "hit_frac": float((hits[:, 1] != L.SPT_MISS).mean()),
+            "mean_tree_units_32B": nodes.mean(), "mean_tris": tris.mean(), "hit_frac": float((hits[:, 1] != L.SPT_MISS).mean()),
             "fetch_gbs": byts / (ms * 1e-3) / 1e9}
 
 
@@ -375,7 +375,7 @@ def main():
     per_ray_nodes = fs["node_fetches"] / fs["rays"]
     per_ray_tris = fs["tri_fetches"] / fs["rays"]
     # per extension ray: queue index 4 + origin 16 + direction 16 in, hit 16 + sort key 4 out
-    bytes_per_ray = 56 + 128 * per_ray_nodes + 48 * per_ray_tris
+    bytes_per_ray = 56 + 32 * per_ray_nodes + 48 * per_ray_tris  # tree data counted in 32-B units
     achieved = bytes_per_ray * rays / (ext_ms * 1e-3) / 1e9
     img = film.download(np.float32, (cam.height, cam.width, 3)) / (args.steps * sps)
 
@@ -428,7 +428,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_extend (closest-hit traversal of extension rays)",
-                     "bytes_per_ray": round(bytes_per_ray, 1), "node_fetches_per_ray": round(per_ray_nodes, 2),
+                     "bytes_per_ray": round(bytes_per_ray, 1), "tree_units_32B_per_ray": round(per_ray_nodes, 2),
                      "tri_fetches_per_ray": round(per_ray_tris, 2),
                      "avg_launch_ms": round(ext_ms / max(ext_n, 1), 4), "launches": ext_n},
         "cpu_baseline": cpu,

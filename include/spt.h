@@ -153,8 +153,9 @@ enum {
     SPT_RENDER_PER_PATH = 1u << 2,  /* write each path's radiance into d_path_rgb */
     SPT_RENDER_NO_SORT = 1u << 3,   /* skip the material sort between stages */
     SPT_RENDER_GUIDED = 1u << 4,    /* sample with the learned guide (spt_guide_*) */
-    SPT_RENDER_FETCH_STATS = 1u << 5, /* count 128-B node / 48-B triangle record fetches
-                                         of the extension rays (slower traversal variant) */
+    SPT_RENDER_FETCH_STATS = 1u << 5, /* count tree data in 32-B units (a 64-B node is 2,
+                                         a 32-B leaf record 1) and 48-B triangle records
+                                         fetched by the extension rays (slower variant) */
     SPT_RENDER_TIME_KERNELS = 1u << 6 /* time every extend launch with HIP events */
 };
 
@@ -216,6 +217,18 @@ spt_status spt_bvh_export(const spt_bvh* bvh, spt_node64* nodes, spt_tri48* tris
                           float* root_box);
 /* Writes the reference's tree dump text format (acceleration/sbvh.cpp:577-620). */
 spt_status spt_bvh_dump(const spt_bvh* bvh, const char* path);
+/* The four-wide compressed tree the throughput traversal reads, for
+ * inspection (no reference counterpart). Node (64 B, 16 u32): grid origin
+ * x y z (f32); biased exponent bytes of the per-axis grid step 2^e (bytes
+ * 0-2); six words of child bytes q (lo.x hi.x lo.y hi.y lo.z hi.z, byte i =
+ * child i; bound = origin + q * 2^e, lo rounded down, hi up); two words of
+ * 16-bit child triangle counts (0: inner or unused, 0xFFFF: see the box
+ * record); four children (node index; first triangle in leaf order for a
+ * leaf; 0xFFFFFFFF unused). Leaf boxes (32 B, four per node, slot i of node
+ * j at 4 j + i): exact box min xyz (f32), full triangle count, box max xyz,
+ * unused. NULL arrays: the counts are returned; otherwise *n_nodes and
+ * *n_leaves give the arrays' capacities on entry. */
+spt_status spt_bvh_wide(const spt_bvh* bvh, void* nodes, uint64_t* n_nodes, void* leaves, uint64_t* n_leaves);
 void spt_bvh_destroy(spt_bvh* bvh);
 
 /* ------------------------------------------------------------------------- */
@@ -258,9 +271,9 @@ enum {
     SPT_TRACE_COUNT = 1,     /* visits the reference's exact node set and also
                                 returns its counters (sbvh.cpp:478-510) */
     SPT_TRACE_UNPRUNED = 2,  /* like COUNT without writing counters */
-    SPT_TRACE_FETCH = 3      /* CLOSEST, and d_tests/d_isects receive the number of
-                                128-B four-wide node records and 48-B triangle
-                                records each ray fetched */
+    SPT_TRACE_FETCH = 3      /* CLOSEST, and d_tests receives the tree data each ray
+                                fetched in 32-B units (a 64-B node is 2, a 32-B
+                                leaf record 1), d_isects its 48-B triangle records */
 };
 
 /* d_rays: n rays; d_hits: n hits. In SPT_TRACE_COUNT mode d_tests/d_isects
@@ -294,7 +307,7 @@ spt_status spt_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_para
 /* Counters of the last spt_render call: paths, extension rays traced, bounce
  * iterations, host wall ms; with SPT_RENDER_TIME_KERNELS the summed device time
  * and count of the extend (closest-hit) launches; with SPT_RENDER_FETCH_STATS the
- * node / triangle records those rays fetched. */
+ * tree data (node_fetches, in 32-B units) and triangle records those rays fetched. */
 typedef struct spt_render_stats {
     uint64_t paths;
     uint64_t rays;

@@ -136,6 +136,7 @@ PROTOTYPES = {
     "spt_bvh_get_info": (C.c_int32, [C.c_void_p, C.POINTER(BvhInfo)]),
     "spt_bvh_export": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "spt_bvh_dump": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "spt_bvh_wide": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.POINTER(C.c_uint64)]),
     "spt_bvh_destroy": (None, [C.c_void_p]),
     "spt_create": (C.c_int32, [C.c_int, C.POINTER(C.c_void_p)]),
     "spt_destroy": (None, [C.c_void_p]),

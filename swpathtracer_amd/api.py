@@ -175,6 +175,20 @@ class Bvh:
             raise SptError(s, "spt_bvh_export failed")
         return nodes, tris, leaf, box
 
+    def wide(self):
+        """The compressed four-wide tree the traversal reads (spt_bvh_wide):
+        nodes (n, 16) uint32 and leaf records (m, 8) uint32 (float fields as
+        their bits)."""
+        nn, nl = C.c_uint64(0), C.c_uint64(0)
+        s = self.lib.spt_bvh_wide(self.h, None, C.byref(nn), None, C.byref(nl))
+        nodes = np.zeros((max(nn.value, 1), 16), np.uint32)
+        leaves = np.zeros((max(nl.value, 1), 8), np.uint32)
+        if s == L.SPT_OK:
+            s = self.lib.spt_bvh_wide(self.h, nodes.ctypes.data, C.byref(nn), leaves.ctypes.data, C.byref(nl))
+        if s != L.SPT_OK:
+            raise SptError(s, "spt_bvh_wide failed")
+        return nodes[:nn.value], leaves[:nl.value]
+
     def dump(self, path: str):
         s = self.lib.spt_bvh_dump(self.h, path.encode())
         if s != L.SPT_OK:

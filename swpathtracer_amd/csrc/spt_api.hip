@@ -2,6 +2,8 @@
 // the entry points that stand in for AccelStructure::intersect and
 // Integrator::render. No exception crosses this boundary.
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -85,12 +87,14 @@ namespace {
 
 void free_scene(spt_ctx* ctx) {
     if (ctx->d_nodes) (void)hipFree(ctx->d_nodes);
-    if (ctx->d_nodes4) (void)hipFree(ctx->d_nodes4);
+    if (ctx->d_nodesq) (void)hipFree(ctx->d_nodesq);
+    if (ctx->d_leaves) (void)hipFree(ctx->d_leaves);
     if (ctx->d_tris) (void)hipFree(ctx->d_tris);
     if (ctx->d_shading) (void)hipFree(ctx->d_shading);
     ctx->d_nodes = nullptr;
-    ctx->d_nodes4 = nullptr;
-    ctx->n_nodes4 = 0;
+    ctx->d_nodesq = nullptr;
+    ctx->d_leaves = nullptr;
+    ctx->n_nodesq = ctx->n_leaves = 0;
     ctx->d_tris = nullptr;
     ctx->d_shading = nullptr;
     ctx->n_nodes = ctx->n_tris = 0;
@@ -246,11 +250,116 @@ uint32_t collapse4(const spt_node64* nodes, uint32_t n, std::vector<Node4>& out)
     return 3 * max_depth + 1;
 }
 
+namespace {
+
+// The value the traversal decodes for byte q (qdec): origin + q * scale in
+// float, q * scale exact.
+float grid_value(float origin, uint32_t q, float scale) {
+    const volatile float p = (float)q * scale;
+    return origin + p;
+}
+
+// One axis of a node: the grid (origin = smallest lo, scale = a power of two
+// with origin + 255 scale >= largest hi) and each child's bytes, lo rounded
+// down and hi rounded up on the grid. Returns the biased exponent byte.
+uint32_t quantize_axis(const float* lo, const float* hi, const bool* used, float& origin, uint32_t& qlo,
+                       uint32_t& qhi) {
+    float mn = std::numeric_limits<float>::infinity(), mx = -std::numeric_limits<float>::infinity();
+    for (int i = 0; i < 4; ++i)
+        if (used[i]) {
+            mn = std::min(mn, lo[i]);
+            mx = std::max(mx, hi[i]);
+        }
+    if (!(mn <= mx)) mn = mx = 0.0f;
+    origin = mn;
+    int e = -126;
+    const float ext = (mx - mn) / 255.0f;
+    if (ext > 0.0f) {
+        (void)std::frexp(ext, &e);  // ext < 2^e
+        e = std::max(e, -126);
+    }
+    while (grid_value(origin, 255u, std::ldexp(1.0f, e)) < mx) ++e;
+    const float s = std::ldexp(1.0f, e);
+    qlo = qhi = 0;
+    for (int i = 0; i < 4; ++i) {
+        if (!used[i]) continue;
+        double fl = std::floor(((double)lo[i] - origin) / s), fh = std::ceil(((double)hi[i] - origin) / s);
+        uint32_t a = (uint32_t)std::min(std::max(fl, 0.0), 255.0), b = (uint32_t)std::min(std::max(fh, 0.0), 255.0);
+        while (a > 0 && grid_value(origin, a, s) > lo[i]) --a;
+        while (b < 255 && grid_value(origin, b, s) < hi[i]) ++b;
+        qlo |= a << (8 * i);
+        qhi |= b << (8 * i);
+    }
+    return (uint32_t)(e + 127);
+}
+
+}  // namespace
+
+bool compress4(const std::vector<Node4>& in, uint32_t tri_base, std::vector<NodeQ>& nodes, std::vector<LeafRec>& leaves) {
+    const uint64_t node_base = nodes.size();
+    // leaf stack entries hold node << 2 | slot below kLeafBit
+    if (node_base + in.size() > (1ull << 29)) return false;
+    leaves.resize(4 * (node_base + in.size()), LeafRec{});
+    for (size_t j = 0; j < in.size(); ++j) {
+        const Node4& n = in[j];
+        NodeQ q{};
+        bool used[4];
+        for (int i = 0; i < 4; ++i) used[i] = n.child[i] != kNoChild;
+        const uint32_t ex = quantize_axis(n.lox, n.hix, used, q.ox, q.q[0], q.q[1]);
+        const uint32_t ey = quantize_axis(n.loy, n.hiy, used, q.oy, q.q[2], q.q[3]);
+        const uint32_t ez = quantize_axis(n.loz, n.hiz, used, q.oz, q.q[4], q.q[5]);
+        q.exps = ex | (ey << 8) | (ez << 16);
+        for (int i = 0; i < 4; ++i) {
+            if (!used[i]) {
+                q.child[i] = kNoChild;
+            } else if (n.count[i] > 0) {
+                if ((uint64_t)n.child[i] + tri_base + n.count[i] > 0xFFFFFFFFull) return false;
+                LeafRec& r = leaves[4 * (node_base + j) + i];
+                r.mn[0] = n.lox[i];
+                r.mn[1] = n.loy[i];
+                r.mn[2] = n.loz[i];
+                r.mx[0] = n.hix[i];
+                r.mx[1] = n.hiy[i];
+                r.mx[2] = n.hiz[i];
+                r.cnt = n.count[i];
+                q.child[i] = n.child[i] + tri_base;
+                q.cnt[i >> 1] |= std::min<uint32_t>(n.count[i], 0xFFFFu) << (16 * (i & 1));
+            } else {
+                q.child[i] = n.child[i] + (uint32_t)node_base;
+            }
+        }
+        nodes.push_back(q);
+    }
+    return true;
+}
+
 }  // namespace spt
 
 using namespace spt;
 
 extern "C" {
+
+spt_status spt_bvh_wide(const spt_bvh* bvh, void* nodes, uint64_t* n_nodes, void* leaves, uint64_t* n_leaves) {
+    if (!bvh || !n_nodes || !n_leaves) return SPT_ERR_INVALID;
+    const std::vector<spt_node64>& bn = bvh_nodes(bvh);
+    std::vector<Node4> n4;
+    (void)collapse4(bn.data(), (uint32_t)bn.size(), n4);
+    std::vector<NodeQ> nq;
+    std::vector<LeafRec> lr;
+    if (!compress4(n4, 0u, nq, lr)) return SPT_ERR_UNSUPPORTED;
+    if (nodes) {
+        if (*n_nodes < nq.size()) return SPT_ERR_INVALID;
+        std::memcpy(nodes, nq.data(), nq.size() * sizeof(NodeQ));
+    }
+    if (leaves) {
+        if (*n_leaves < lr.size()) return SPT_ERR_INVALID;
+        std::memcpy(leaves, lr.data(), lr.size() * sizeof(LeafRec));
+    }
+    *n_nodes = nq.size();
+    *n_leaves = lr.size();
+    return SPT_OK;
+}
+
 
 spt_status spt_create(int device, spt_ctx** out) {
     if (!out) return SPT_ERR_INVALID;
@@ -388,8 +497,12 @@ spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_no
     free_scene(ctx);
     std::vector<Node4> n4;
     const uint32_t cap4 = collapse4(nodes, n_nodes, n4);
+    std::vector<NodeQ> nq;
+    std::vector<LeafRec> leaves;
+    if (!compress4(n4, 0u, nq, leaves)) return set_error(ctx, SPT_ERR_INVALID, "tree over 2^29 four-wide nodes");
     spt_status s = upload(ctx, &ctx->d_nodes, nodes, (size_t)n_nodes * sizeof(spt_node64), "nodes");
-    if (s == SPT_OK) s = upload(ctx, &ctx->d_nodes4, n4.data(), n4.size() * sizeof(Node4), "four-wide nodes");
+    if (s == SPT_OK) s = upload(ctx, &ctx->d_nodesq, nq.data(), nq.size() * sizeof(NodeQ), "compressed nodes");
+    if (s == SPT_OK) s = upload(ctx, &ctx->d_leaves, leaves.data(), leaves.size() * sizeof(LeafRec), "leaf records");
     if (s == SPT_OK) s = upload(ctx, &ctx->d_tris, tris, (size_t)n_tris * sizeof(spt_tri48), "triangles");
     std::vector<float> flat;
     if (s == SPT_OK) {
@@ -420,7 +533,8 @@ spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_no
     std::memcpy(ctx->root_box, root_box, sizeof(ctx->root_box));
     ctx->root_leaf = n_nodes == 0 ? n_tris : 0;
     ctx->stack_cap = tree_depth(nodes, n_nodes) + 1;
-    ctx->n_nodes4 = (uint32_t)n4.size();
+    ctx->n_nodesq = (uint32_t)nq.size();
+    ctx->n_leaves = (uint32_t)leaves.size();
     ctx->stack_cap4 = cap4;
     ctx->max_mat = 0;
     for (uint32_t i = 0; i < n_tris; ++i) ctx->max_mat = std::max(ctx->max_mat, tris[i].pad[0]);

@@ -15,8 +15,8 @@ namespace spt {
 namespace {
 
 void free_batch(BatchData& b) {
-    void* ptrs[] = {b.d_nodes, b.d_nodes4, b.d_tris,       b.d_shading, b.d_mats,
-                    b.d_jobs,  b.d_tiles,  b.d_tile_start, b.d_done,    b.d_clock};
+    void* ptrs[] = {b.d_nodes, b.d_nodesq, b.d_leaves,     b.d_tris, b.d_shading, b.d_mats,
+                    b.d_jobs,  b.d_tiles,  b.d_tile_start, b.d_done, b.d_clock};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     b = BatchData{};
@@ -164,7 +164,8 @@ spt_status spt_batch_create(spt_ctx* ctx, const spt_batch_scene* scenes, uint32_
     if (!ctx || !scenes || n == 0 || !out) return SPT_ERR_INVALID;
     *out = nullptr;
     std::vector<spt_node64> nodes;
-    std::vector<Node4> nodes4;
+    std::vector<NodeQ> nodesq;
+    std::vector<LeafRec> leaves;
     std::vector<spt_tri48> tris;
     std::vector<float> sh;
     std::vector<spt_material> mats;
@@ -208,14 +209,9 @@ spt_status spt_batch_create(spt_ctx* ctx, const spt_batch_scene* scenes, uint32_
         }
         std::vector<Node4> b4;
         d.stack_cap4 = std::max(d.stack_cap4, collapse4(bn.data(), (uint32_t)bn.size(), b4));
-        const uint64_t node4_base = nodes4.size();
-        if (node4_base + b4.size() >= (1u << 29))
-            return fail(set_error(ctx, SPT_ERR_UNSUPPORTED, "batch exceeds 2^29 four-wide nodes"));
-        for (Node4 q : b4) {
-            for (int k = 0; k < 4; ++k)
-                if (q.child[k] != kNoChild) q.child[k] += q.count[k] ? (uint32_t)tri_base : (uint32_t)node4_base;
-            nodes4.push_back(q);
-        }
+        const uint64_t node4_base = nodesq.size();
+        if (!compress4(b4, (uint32_t)tri_base, nodesq, leaves))
+            return fail(set_error(ctx, SPT_ERR_UNSUPPORTED, "batch exceeds 2^31 four-wide nodes or leaves"));
         for (spt_tri48 t : bt) {
             t.pad[0] += (uint32_t)mat_base;
             tris.push_back(t);
@@ -235,7 +231,7 @@ spt_status spt_batch_create(spt_ctx* ctx, const spt_batch_scene* scenes, uint32_
             jb.rmax[k] = rb[3 + k];
         }
         jb.root_leaf = bn.empty() ? (uint32_t)bt.size() : 0u;
-        jb.node0 = (uint32_t)node4_base;  // persistent traversal starts at the scene's Node4 root
+        jb.node0 = (uint32_t)node4_base;  // persistent traversal starts at the scene's NodeQ root
         jb.tri0 = (uint32_t)tri_base;
         d.stack_cap = std::max(d.stack_cap, tree_stack_cap(bn.data(), (uint32_t)bn.size()));
         d.jobs.push_back(jb);
@@ -243,7 +239,8 @@ spt_status spt_batch_create(spt_ctx* ctx, const spt_batch_scene* scenes, uint32_
     (void)hipSetDevice(ctx->device);
     uint64_t bytes = 0;
     spt_status s = upload_vec(ctx, reinterpret_cast<spt_node64**>(&d.d_nodes), nodes, "batch nodes", bytes);
-    if (s == SPT_OK) s = upload_vec(ctx, reinterpret_cast<Node4**>(&d.d_nodes4), nodes4, "batch four-wide nodes", bytes);
+    if (s == SPT_OK) s = upload_vec(ctx, &d.d_nodesq, nodesq, "batch compressed nodes", bytes);
+    if (s == SPT_OK) s = upload_vec(ctx, &d.d_leaves, leaves, "batch leaf records", bytes);
     if (s == SPT_OK) s = upload_vec(ctx, reinterpret_cast<spt_tri48**>(&d.d_tris), tris, "batch triangles", bytes);
     if (s == SPT_OK) s = upload_vec(ctx, reinterpret_cast<float**>(&d.d_shading), sh, "batch shading", bytes);
     if (s == SPT_OK) s = upload_vec(ctx, &d.d_mats, mats, "batch materials", bytes);

@@ -28,6 +28,39 @@ struct Node4 {
 static_assert(sizeof(Node4) == 128, "Node4 layout");
 constexpr uint32_t kNoChild = 0xFFFFFFFFu;
 
+// Compressed four-wide node (64 B, what the throughput traversal reads):
+// each axis has a per-node grid origin + q * 2^e, and every child bound is
+// rounded outward to a byte q (lo down, hi up), so the decoded boxes contain
+// the exact padded child boxes (the traversal visits a superset of the exact
+// four-wide tree's boxes; results do not depend on visit order).
+//   word 0-2: grid origin x y z (float); word 3: biased exponent bytes of
+//   2^e_x, 2^e_y, 2^e_z (float exponent field, bytes 0-2)
+//   words 4-9: q of lo.x, hi.x, lo.y, hi.y, lo.z, hi.z (byte i = child i)
+//   words 10-11: 16-bit triangle counts of the children (0: inner or unused;
+//   0xFFFF: 65535 or more, the full count is in the leaf's box record)
+//   words 12-15: children: NodeQ index (inner), first triangle (leaf), kNoChild
+// A leaf's range is read from the node that holds it, which the traversal
+// has just visited; its exact box (the reference's hit-point-inside-leaf
+// test) sits in a side array at 4 * node + slot and is read only when a
+// triangle of the leaf passes the t tests.
+struct NodeQ {
+    float ox, oy, oz;
+    uint32_t exps;
+    uint32_t q[6];
+    uint32_t cnt[2];
+    uint32_t child[4];
+};
+static_assert(sizeof(NodeQ) == 64, "NodeQ layout");
+
+// Exact padded box of the leaf in slot s of node i, at index 4 i + s (32 B).
+struct LeafRec {
+    float mn[3];
+    uint32_t cnt;  // full triangle count
+    float mx[3];
+    uint32_t pad;
+};
+static_assert(sizeof(LeafRec) == 32, "LeafRec layout");
+
 // Materials the learned guide samples: one Lambertian / importance-sampled
 // GGX layer (kGuideSingle), or such a layer under one smooth dielectric coat
 // of IOR >= 1 (kGuideCoated: the guide drives the base layer's directions).
@@ -35,7 +68,8 @@ constexpr uint8_t kGuideSingle = 1, kGuideCoated = 2;
 
 // Scene as the kernels see it.
 struct DevScene {
-    const float4* nodes4;   // n_nodes4 x 8 float4 (Node4), persistent traversal
+    const uint4* nodesq;    // NodeQ x 4 uint4, persistent traversal
+    const float4* leaves;   // LeafRec x 2 float4, 4 per node (leaf slots used)
     const float4* nodes;    // n_nodes x 4 float4 (spt_node64)
     const float4* tris;     // n_tris x 3 float4 (spt_tri48; .pad[0] = material id)
     const float4* shading;  // n_tris x 3 float4: n0 n1 n2, material, smooth flag, pad
@@ -200,12 +234,13 @@ __device__ inline void trace_ray(const DevScene& sc, V3 o, V3 d, Stack& st, Hit&
 // with one wave-aggregated atomic, then every lane runs an inner-node phase
 // (descend near-first, push the far child) followed by a leaf phase (test the
 // triangles of postponed and consecutive leaves). Leaves are stack entries
-// too, encoded as kLeafBit | node << 1 | side. The inner phase is
+// too, encoded as kLeafBit | node << 2 | slot. The inner phase is
 // speculative (Aila and Laine, "Understanding the efficiency of ray traversal
 // on GPUs", 2009): a lane that meets its first leaf postpones it and keeps
 // descending until every lane holds a leaf, so lanes wait less for each
-// other. The leaf's box (needed for the reference's hit-point-inside-leaf
-// test) is reread from its parent node, which is cache-resident by then.
+// other. Nodes are the compressed NodeQ; a leaf's range is reread from its
+// node and its exact box (the reference's hit-point-inside-leaf test) is
+// read only when one of its triangles passes the t tests.
 // Same candidate acceptance and tie rule as trace_ray<true, false>: the
 // closest hit is the minimum of (t, leaf index), independent of visit order,
 // and boxes are only skipped when they start beyond the best t found so far.
@@ -231,23 +266,40 @@ __device__ inline float safe_rcp(float d) {
     return (fabsf(d) > 1e-30f) ? 1.0f / d : copysignf(1e30f, d);
 }
 
-// One child of a Node4 against the ray: key = entry distance (+inf when the
-// box is missed, lies behind the origin or beyond the best hit t_best, or the
-// slot is unused), entry = stack entry of the child. The test is one FMA per
-// slab plane and IEEE min/max (a NaN product keeps the box): conservative, it
-// may visit a box the reference's exact slab test skips but never skips one
-// holding a closer candidate, since every box is padded 1e-3 around its
-// triangles (acceleration/sbvh.cpp:409-412).
-__device__ inline void child_test(V3 inv, V3 oi, float t_best, float lox, float hix, float loy, float hiy, float loz,
-                                  float hiz, uint32_t child, uint32_t cnt, uint32_t node, uint32_t slot, float& key,
-                                  uint32_t& entry) {
+// One child box against the ray: key = entry distance (+inf when the box is
+// missed, lies behind the origin or beyond the best hit t_best, or the slot
+// is unused). The test is one FMA per slab plane and IEEE min/max (a NaN
+// product keeps the box): conservative, it may visit a box the reference's
+// exact slab test skips but never skips one holding a closer candidate,
+// since every box is padded 1e-3 around its triangles
+// (acceleration/sbvh.cpp:409-412) and the compressed boxes contain the
+// padded ones.
+__device__ inline float child_test(V3 inv, V3 oi, float t_best, float lox, float hix, float loy, float hiy, float loz,
+                                   float hiz, uint32_t child) {
     const float t0x = fmaf(lox, inv.x, oi.x), t1x = fmaf(hix, inv.x, oi.x);
     const float t0y = fmaf(loy, inv.y, oi.y), t1y = fmaf(hiy, inv.y, oi.y);
     const float t0z = fmaf(loz, inv.z, oi.z), t1z = fmaf(hiz, inv.z, oi.z);
     const float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
     const float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), t_best));
-    key = (child != kNoChild && tmin <= tmax) ? tmin : __builtin_inff();
-    entry = cnt ? (kLeafBit | (node << 2) | slot) : child;
+    return (child != kNoChild && tmin <= tmax) ? tmin : __builtin_inff();
+}
+
+// Byte b of word w as a float.
+__device__ inline float qbyte(uint32_t w, int b) { return (float)((w >> (8 * b)) & 0xFFu); }
+
+// Slab test of one compressed child in ray space: for a plane at origin +
+// q * scale, t = (origin + q * scale - o) / d = q * (scale / d) + (origin / d
+// - o / d), one FMA per plane with a = scale * inv and b = origin * inv + oi
+// per node and axis. Rounding differs from the world-space form by a few
+// ulps of the same terms, far inside the 1e-3 box padding.
+__device__ inline float child_test_q(float ax, float bx, float ay, float by, float az, float bz, float t_best,
+                                     float qlx, float qhx, float qly, float qhy, float qlz, float qhz, uint32_t child) {
+    const float t0x = fmaf(qlx, ax, bx), t1x = fmaf(qhx, ax, bx);
+    const float t0y = fmaf(qly, ay, by), t1y = fmaf(qhy, ay, by);
+    const float t0z = fmaf(qlz, az, bz), t1z = fmaf(qhz, az, bz);
+    const float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+    const float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), t_best));
+    return (child != kNoChild && tmin <= tmax) ? tmin : __builtin_inff();
 }
 
 __device__ inline void cswap(float k[4], uint32_t e[4], int a, int b) {
@@ -312,10 +364,8 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                     // a ray with a non-finite component cannot hit anything (every
                     // t comparison fails) but would pass every box test: a miss
                     const float sum = L.o.x + L.o.y + L.o.z + L.d.x + L.d.y + L.d.z;
-                    float troot;
-                    uint32_t unused;
-                    child_test(L.inv4, L.oi, __builtin_inff(), R.mn.x, R.mx.x, R.mn.y, R.mx.y, R.mn.z, R.mx.z, 0u, 0u,
-                               0u, 0u, troot, unused);
+                    const float troot = child_test(L.inv4, L.oi, __builtin_inff(), R.mn.x, R.mx.x, R.mn.y, R.mx.y,
+                                                   R.mn.z, R.mx.z, 0u);
                     if (troot == __builtin_inff() || !(sum - sum == 0.0f))
                         L.cur = kDone;
                     else
@@ -332,18 +382,24 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
         // still searching for a leaf (or none has inner work left)
         while (true) {
             if (L.ray != kDone && L.cur != kDone && !(L.cur & kLeafBit)) {
-                const uint32_t node = L.cur;
-                const float4* np = sc.nodes4 + 8 * (size_t)node;
-                const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-                const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
-                const uint4 cn = reinterpret_cast<const uint4*>(np)[7];
-                if (FETCH) ++L.nodes;
+                const uint4* np = sc.nodesq + 4 * (size_t)L.cur;
+                const uint4 g = np[0], qa = np[1], qb = np[2], ch = np[3];
+                if (FETCH) L.nodes += 2;  // 32-B units
+                const float ox = __uint_as_float(g.x), oy = __uint_as_float(g.y), oz = __uint_as_float(g.z);
+                const float sx = __uint_as_float((g.w & 0xFFu) << 23), sy = __uint_as_float(((g.w >> 8) & 0xFFu) << 23),
+                            sz = __uint_as_float(((g.w >> 16) & 0xFFu) << 23);
+                const float ax = sx * L.inv4.x, ay = sy * L.inv4.y, az = sz * L.inv4.z;
+                const float bx = fmaf(ox, L.inv4.x, L.oi.x), by = fmaf(oy, L.inv4.y, L.oi.y),
+                            bz = fmaf(oz, L.inv4.z, L.oi.z);
                 float k[4];
-                uint32_t e[4];
-                child_test(L.inv4, L.oi, L.h.t, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x, cn.x, node, 0, k[0], e[0]);
-                child_test(L.inv4, L.oi, L.h.t, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y, cn.y, node, 1, k[1], e[1]);
-                child_test(L.inv4, L.oi, L.h.t, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z, cn.z, node, 2, k[2], e[2]);
-                child_test(L.inv4, L.oi, L.h.t, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w, cn.w, node, 3, k[3], e[3]);
+                uint32_t e[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    k[c] = child_test_q(ax, bx, ay, by, az, bz, L.h.t, qbyte(qa.x, c), qbyte(qa.y, c), qbyte(qa.z, c),
+                                        qbyte(qa.w, c), qbyte(qb.x, c), qbyte(qb.y, c), e[c]);
+                    const uint32_t cw = (c < 2) ? qb.z : qb.w;
+                    if ((cw >> (16 * (c & 1))) & 0xFFFFu) e[c] = kLeafBit | (L.cur << 2) | (uint32_t)c;
+                }
                 // nearest first: sort the four (entry, entry distance) pairs, misses (+inf) last
                 cswap(k, e, 0, 1);
                 cswap(k, e, 2, 3);
@@ -385,23 +441,45 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
             if (!__any(has)) break;
             if (has) {
                 uint32_t first, cnt;
-                V3 bmn, bmx;
+                const float4* box = nullptr;  // leaf box record, read when a candidate needs it
+                V3 bmn = R.mn, bmx = R.mx;
                 if (R.leaf) {
                     first = R.tri0;
                     cnt = R.leaf;
-                    bmn = R.mn;
-                    bmx = R.mx;
                 } else {
-                    // the leaf's box and range from its Node4 (cache-resident by now)
-                    const uint32_t node = (L.leaf & ~kLeafBit) >> 2, slot = L.leaf & 3u;
-                    const float* f = reinterpret_cast<const float*>(sc.nodes4 + 8 * (size_t)node) + slot;
-                    bmn = V3{f[0], f[8], f[16]};
-                    bmx = V3{f[4], f[12], f[20]};
-                    first = __float_as_uint(f[24]);
-                    cnt = __float_as_uint(f[28]);
+                    // the range from the node that holds the leaf (visited just before)
+                    const uint32_t slot = L.leaf & 3u, li = L.leaf & ~kLeafBit;
+                    const uint32_t* np = reinterpret_cast<const uint32_t*>(sc.nodesq + 4 * (size_t)(li >> 2));
+                    first = np[12 + slot];
+                    cnt = (np[10 + (slot >> 1)] >> (16 * (slot & 1u))) & 0xFFFFu;
+                    box = sc.leaves + 2 * (size_t)li;
+                    if (cnt == 0xFFFFu) cnt = __float_as_uint(box[0].w);
                 }
-                uint32_t dummy_t = 0, dummy_i = 0;
-                leaf_test<false, false>(sc, L.o, L.d, first, cnt, bmn, bmx, L.h, dummy_t, dummy_i);
+                bool have_box = R.leaf != 0u;
+                for (uint32_t k = first; k < first + cnt; ++k) {
+                    const float4* tp = sc.tris + 3 * (size_t)k;
+                    const float4 a = tp[0], b = tp[1], c = tp[2];
+                    float t, u, v;
+                    if (tri_test(L.o, L.d, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, t, u, v) &&
+                        (t > 0.0f) && ((t < L.h.t) || ((t == L.h.t) && (k < L.h.idx)))) {
+                        if (!have_box) {
+                            const float4 b0 = box[0], b1 = box[1];
+                            bmn = V3{b0.x, b0.y, b0.z};
+                            bmx = V3{b1.x, b1.y, b1.z};
+                            have_box = true;
+                            if (FETCH) ++L.nodes;
+                        }
+                        // leaf_test's acceptance (sbvh.cpp:494-496)
+                        if (point_inside(L.o + L.d * t, bmn, bmx)) {
+                            L.h.t = t;
+                            L.h.idx = k;
+                            L.h.prim = __float_as_uint(c.y);
+                            L.h.mat = __float_as_uint(c.z);
+                            L.h.u = u;
+                            L.h.v = v;
+                        }
+                    }
+                }
                 if (FETCH) L.tris += cnt;
                 L.leaf = kDone;
             }
@@ -491,7 +569,8 @@ struct BatchTile {
 // Device arrays and per-scene jobs of a batch (spt_batch_*).
 struct BatchData {
     float4* d_nodes = nullptr;
-    float4* d_nodes4 = nullptr;
+    NodeQ* d_nodesq = nullptr;
+    LeafRec* d_leaves = nullptr;
     float4* d_tris = nullptr;
     float4* d_shading = nullptr;
     spt_material* d_mats = nullptr;
@@ -601,8 +680,9 @@ struct spt_ctx {
     int num_cus = 256;
     // scene
     float4* d_nodes = nullptr;
-    float4* d_nodes4 = nullptr;
-    uint32_t n_nodes4 = 0, stack_cap4 = 0;
+    spt::NodeQ* d_nodesq = nullptr;
+    spt::LeafRec* d_leaves = nullptr;
+    uint32_t n_nodesq = 0, n_leaves = 0, stack_cap4 = 0;
     float4* d_tris = nullptr;
     float4* d_shading = nullptr;
     spt_material* d_mats = nullptr;
@@ -636,7 +716,8 @@ struct spt_ctx {
     spt::DevScene scene() const {
         spt::DevScene s;
         s.nodes = d_nodes;
-        s.nodes4 = d_nodes4;
+        s.nodesq = reinterpret_cast<const uint4*>(d_nodesq);
+        s.leaves = reinterpret_cast<const float4*>(d_leaves);
         s.stack_cap4 = stack_cap4;
         s.tris = d_tris;
         s.shading = d_shading;
@@ -680,6 +761,10 @@ uint32_t tree_stack_cap(const spt_node64* nodes, uint32_t n);
 // Collapses a pre-order pair-node tree into four-wide nodes (pre-order, root
 // first); returns the stack entries the persistent traversal needs.
 uint32_t collapse4(const spt_node64* nodes, uint32_t n, std::vector<Node4>& out);
+// Appends the compressed form of a four-wide tree (and four leaf box slots
+// per node): inner children rebased by the node array's size, leaf ranges by
+// tri_base. Returns false when an index would not fit its field.
+bool compress4(const std::vector<Node4>& in, uint32_t tri_base, std::vector<NodeQ>& nodes, std::vector<LeafRec>& leaves);
 spt_status make_job(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* prm, float* d_rgb32, double* d_rgb64,
                     float* d_path_rgb, Job& jb, uint64_t& total);
 // A multi-scene render (spt_batch_*): every job of the batch, or with

@@ -1165,7 +1165,8 @@ spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div) {
 
     RenderArgs a{};
     a.sc.nodes = b.d_nodes;
-    a.sc.nodes4 = b.d_nodes4;
+    a.sc.nodesq = reinterpret_cast<const uint4*>(b.d_nodesq);
+    a.sc.leaves = reinterpret_cast<const float4*>(b.d_leaves);
     a.sc.stack_cap4 = b.stack_cap4;
     a.sc.tris = b.d_tris;
     a.sc.shading = b.d_shading;

@@ -131,3 +131,77 @@ def test_call_paths_counts_the_rounded_sample_grid():
     p.spp_count = 3
     p.viewport = (C.c_uint32 * 4)(1, 2, 3, 2)
     assert api.Device.call_paths(cam, p) == 6 * 3
+
+
+def _decode_wide(nodes):
+    """Decoded child bounds (n, 4, 2, 3) as the traversal computes them:
+    float32 origin + q * 2^e (q * 2^e exact, one rounding in the add)."""
+    org = nodes[:, 0:3].view(np.float32)
+    ebytes = np.stack([(nodes[:, 3] >> (8 * a)) & 0xFF for a in range(3)], 1)
+    scale = (ebytes.astype(np.uint32) << 23).view(np.float32)  # 2^(byte - 127)
+    q = np.stack([(nodes[:, 4:10] >> (8 * c)) & 0xFF for c in range(4)], 1).astype(np.float32)  # (n, 4, 6)
+    out = np.empty((len(nodes), 4, 2, 3), np.float32)
+    for a in range(3):
+        for side in range(2):
+            out[:, :, side, a] = org[:, None, a] + q[:, :, 2 * a + side] * scale[:, None, a]
+    return out, scale
+
+
+@pytest.mark.parametrize("name", ["two", "duplicates", "coplanar_grid", "random_soup", "cornell", "displaced",
+                                  "far_and_small"])
+def test_compressed_wide_tree_contains_the_exact_boxes(name):
+    """The 64-B nodes the traversal reads: every decoded child box (float32
+    decode, as on the device) contains the exact padded boxes of all leaves
+    below it and is at most one grid step looser on each side of a leaf's
+    box; the leaves (range in the node, exact box at 4 node + slot) are
+    exactly the binary tree's leaves; every node is reached once from the
+    root."""
+    if name == "far_and_small":  # tiny triangles far from the origin: grid steps near float spacing
+        v = np.random.default_rng(3).uniform(-1e-2, 1e-2, (2000, 9)) + np.array([3e4, -7e3, 1e5] * 3)
+        bvh = api.Bvh(v.astype(np.float32))
+    else:
+        bvh = api.Bvh(_scenes()[name])
+    bnodes, _, _, _ = bvh.export()
+    nodes, boxes = bvh.wide()
+    if len(bnodes) == 0:  # the root is a leaf: no nodes
+        assert len(nodes) == 0 and len(boxes) == 0
+        return
+    assert len(boxes) == 4 * len(nodes)
+    exp = set()
+    for nd in bnodes:
+        for s in ("l", "r"):
+            if nd[s + "cnt"] > 0:
+                exp.add((tuple(nd[s + "min"].tolist()), tuple(nd[s + "max"].tolist()), int(nd[s + "ref"]),
+                         int(nd[s + "cnt"])))
+    bmn, bmx = boxes[:, 0:3].view(np.float32), boxes[:, 4:7].view(np.float32)
+    dec, scale = _decode_wide(nodes)
+    seen_nodes = np.zeros(len(nodes), int)
+    got = []
+
+    def visit(i):  # returns the union of leaf boxes below node i
+        seen_nodes[i] += 1
+        lo, hi = np.full(3, np.inf, np.float32), np.full(3, -np.inf, np.float32)
+        for c in range(4):
+            e = int(nodes[i, 12 + c])
+            cnt16 = (int(nodes[i, 10 + c // 2]) >> (16 * (c & 1))) & 0xFFFF
+            if e == 0xFFFFFFFF:
+                assert cnt16 == 0
+                continue
+            if cnt16:
+                j = 4 * i + c
+                cnt = int(boxes[j, 3])
+                assert cnt16 == min(cnt, 0xFFFF)
+                got.append((tuple(bmn[j].tolist()), tuple(bmx[j].tolist()), e, cnt))
+                clo, chi = bmn[j], bmx[j]
+                # tight: at most one grid step (and the add's rounding) outside the exact leaf box
+                step = scale[i] * np.float32(1.0001)
+                assert np.all(dec[i, c, 0] >= clo - step) and np.all(dec[i, c, 1] <= chi + step)
+            else:
+                clo, chi = visit(e)
+            assert np.all(dec[i, c, 0] <= clo) and np.all(dec[i, c, 1] >= chi), (name, i, c)
+            lo, hi = np.minimum(lo, clo), np.maximum(hi, chi)
+        return lo, hi
+
+    visit(0)
+    assert np.all(seen_nodes == 1)
+    assert len(got) == len(exp) and set(got) == exp
