@@ -456,9 +456,19 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                     if (cnt == 0xFFFFu) cnt = __float_as_uint(box[0].w);
                 }
                 bool have_box = R.leaf != 0u;
-                for (uint32_t k = first; k < first + cnt; ++k) {
-                    const float4* tp = sc.tris + 3 * (size_t)k;
-                    const float4 a = tp[0], b = tp[1], c = tp[2];
+                // software-pipelined: triangle k + 1 is requested before k is tested
+                // (every leaf holds at least one triangle)
+                const uint32_t end = first + cnt;
+                const float4* tp = sc.tris + 3 * (size_t)first;
+                float4 na = tp[0], nb = tp[1], nc = tp[2];
+                for (uint32_t k = first; k < end; ++k) {
+                    const float4 a = na, b = nb, c = nc;
+                    if (k + 1 < end) {
+                        tp += 3;
+                        na = tp[0];
+                        nb = tp[1];
+                        nc = tp[2];
+                    }
                     float t, u, v;
                     if (tri_test(L.o, L.d, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, t, u, v) &&
                         (t > 0.0f) && ((t < L.h.t) || ((t == L.h.t) && (k < L.h.idx)))) {
