@@ -386,6 +386,15 @@ spt_status spt_create(int device, spt_ctx** out) {
         delete ctx;
         return SPT_ERR_DEVICE;
     }
+    // status words the bounce loop writes for the host (mapped host memory)
+    void* hs = nullptr;
+    if (hipHostMalloc(&hs, 256, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) hs = nullptr;
+    ctx->h_status = static_cast<volatile uint32_t*>(hs);
+    if (!hs || hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->d_status), hs, 0) != hipSuccess) {
+        spt_destroy(ctx);
+        return SPT_ERR_DEVICE;
+    }
+    std::memset(hs, 0, 256);
     for (hipEvent_t& e : ctx->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             spt_destroy(ctx);
@@ -411,6 +420,7 @@ void spt_destroy(spt_ctx* ctx) {
     if (ctx->guide.d_scratch) (void)hipFree(ctx->guide.d_scratch);
     ctx->guide.destroy_events();
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
+    if (ctx->h_status) (void)hipHostFree(const_cast<uint32_t*>(ctx->h_status));
     if (ctx->d_guidable) (void)hipFree(ctx->d_guidable);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

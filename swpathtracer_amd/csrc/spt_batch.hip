@@ -75,7 +75,7 @@ spt_status tune(spt_ctx* ctx, bool batch, F run, spt_tuning* out) {
         std::vector<uint32_t> values;
     };
     std::vector<Dim> dims = {
-        {&spt_tuning::path_capacity, {1u << 20, 1u << 21, 1u << 22, 1u << 23}},
+        {&spt_tuning::path_capacity, {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26}},
         {&spt_tuning::extend_blocks_per_cu, {0u, 6u, 4u, 3u, 2u}},
         {&spt_tuning::shade_blocks_per_cu, {4u, 8u, 16u}},
         {&spt_tuning::use_graph, {2u, 1u}},

@@ -722,6 +722,9 @@ struct spt_ctx {
     spt_tuning tuning{};  // zero fields: defaults
     int wall_clock_khz = 100000;
     void* h_pin = nullptr;  // 4 KiB of pinned host memory for small read-backs
+    // mapped host memory the bounce loop reports its progress to (see k_plan)
+    volatile uint32_t* h_status = nullptr;
+    uint32_t* d_status = nullptr;
 
     spt::DevScene scene() const {
         spt::DevScene s;

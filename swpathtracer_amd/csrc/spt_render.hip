@@ -143,21 +143,12 @@ __device__ inline Draws make_draws(const Job& jb, uint64_t key, uint32_t k) {
     return r;
 }
 
-// Camera samples for paths [base, base + n) into buffer A:
+// Camera sample of path g into slot s of buffer A:
 // integrators/pathtracer.cpp:104-113. A single-scene call numbers its paths
 // pixel-major over the viewport; a multi-scene call walks its tile list.
-// With max depth 0 (single-scene calls only) every path finishes here.
+// With max depth 0 (single-scene calls only) the path finishes here.
 template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_generate(RenderArgs a, PathBuf A, Work w, uint64_t base, uint32_t n) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s == 0) {
-        w.cnt[0] = (!MULTI && a.job.max_depth == 0) ? 0u : n;
-        w.cnt[1] = 0;
-        w.cnt[8] = 0;
-        if (base == 0 && a.job_clock) a.job_clock[0] = wall_clock64();
-    }
-    if (s >= n) return;
-    const uint64_t g = base + s;
+__device__ inline void gen_path(const RenderArgs& a, PathBuf& A, const Work& w, uint32_t s, uint64_t g) {
     uint32_t jid = 0;
     uint64_t pl, local;
     uint32_t smp;
@@ -205,6 +196,64 @@ __global__ __launch_bounds__(kBlock) void k_generate(RenderArgs a, PathBuf A, Wo
     A.key[s] = key;
     A.local[s] = local;
     if (a.train) A.grec[s] = make_uint4(kNoRec, kNoRec, kNoRec, 0u);
+}
+
+// Paths [base, base + n) into slots [0, n) of buffer A; a call with max depth
+// 0 finishes them all here.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_generate(RenderArgs a, PathBuf A, Work w, uint64_t base, uint32_t n) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < n) gen_path<MULTI>(a, A, w, s, base + s);
+}
+
+// Work-word layout (w.cnt, 64 words): [0], [1] live counts of the two
+// buffers; [2] replay overflows; [8] ray-fetch counter of the extend kernel;
+// u64 [10] rays, u64 [12] bounces; [16] guide records, [17] candidates;
+// u64 [20] next path to generate; [24] first slot and [25] number of the
+// planned new paths, u64 [26] their first path index; [28] plans made.
+constexpr int kGen = 20, kPlanSlot = 24, kPlanN = 25, kPlanPath = 26, kPlanSeq = 28;
+
+// Between bounces (one thread): the consumed buffer's count and the
+// ray-fetch counter go back to zero (a kernel, not a memset, so the bounce
+// can be graph-captured), and new camera paths are planned into the free
+// tail of the survivors' buffer B: n = min(capacity - survivors, paths not
+// yet generated). Progress goes to mapped host memory: status[1] = live
+// paths after the refill, status[2] = all paths generated, then status[0] =
+// the plan's sequence number, so the host can stop issuing bounces without
+// waiting on the stream.
+__global__ void k_plan(uint32_t* cntA, uint32_t* cntB, uint32_t* cnt, uint32_t cap, uint64_t total,
+                       uint32_t* status, uint64_t* job_clock) {
+    if (threadIdx.x != 0) return;
+    if (cntA) *cntA = 0;
+    cnt[8] = 0;
+    const uint32_t have = *cntB;
+    uint64_t* gen = reinterpret_cast<uint64_t*>(cnt + kGen);
+    const uint64_t g0 = *gen;
+    if (g0 == 0 && job_clock) job_clock[0] = wall_clock64();
+    const uint32_t n = (uint32_t)min((uint64_t)(cap - have), total - g0);
+    cnt[kPlanSlot] = have;
+    cnt[kPlanN] = n;
+    *reinterpret_cast<uint64_t*>(cnt + kPlanPath) = g0;
+    *cntB = have + n;
+    *gen = g0 + n;
+    const uint32_t seq = cnt[kPlanSeq] + 1;
+    cnt[kPlanSeq] = seq;
+    if (status) {
+        volatile uint32_t* hs = status;
+        hs[1] = have + n;
+        hs[2] = (g0 + n == total) ? 1u : 0u;
+        __threadfence_system();
+        hs[0] = seq;
+    }
+}
+
+// The paths k_plan planned, generated into buffer B (grid-stride).
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_refill(RenderArgs a, PathBuf B, Work w) {
+    const uint32_t slot0 = w.cnt[kPlanSlot], n = w.cnt[kPlanN];
+    const uint64_t g0 = *reinterpret_cast<const uint64_t*>(w.cnt + kPlanPath);
+    for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock)
+        gen_path<MULTI>(a, B, w, slot0 + s, g0 + s);
 }
 
 // Closest hit of every live path of buffer A (count *cntA). Persistent
@@ -702,14 +751,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderArgs a, PathBuf A, PathB
     }
 }
 
-// End of a bounce: the consumed buffer's count and the ray-fetch counter go
-// back to zero (a kernel, not a memset, so the bounce can be graph-captured).
-__global__ void k_flip(uint32_t* cntA, uint32_t* fetch) {
-    if (threadIdx.x == 0) {
-        *cntA = 0;
-        *fetch = 0;
-    }
-}
 
 uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
@@ -722,7 +763,7 @@ size_t guide_paths_lds_bytes() {
 // Defaults for the zero fields of a tuning.
 spt_tuning resolve_tuning(const spt_tuning& t) {
     spt_tuning r = t;
-    if (r.path_capacity == 0) r.path_capacity = 1u << 23;
+    if (r.path_capacity == 0) r.path_capacity = 1u << 26;
     if (r.shade_blocks_per_cu == 0) r.shade_blocks_per_cu = 8;
     if (r.use_graph == 0) r.use_graph = 2;
     if (r.tile_pixels == 0) r.tile_pixels = 4096;
@@ -859,6 +900,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
     if (tn.extend_blocks_per_cu) per_cu = std::min<int>(per_cu, (int)tn.extend_blocks_per_cu);
     const uint32_t ext_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)(per_cu * ctx->num_cus));
     const uint32_t shade_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * tn.shade_blocks_per_cu);
+    const uint32_t refill_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * 8);
     spt_status s = ensure_spill(ctx, (size_t)ext_grid * kBlock *
                                          (a.sc.stack_cap4 > kLdsStack ? a.sc.stack_cap4 - kLdsStack : 1));
     if (s != SPT_OK) return s;
@@ -936,7 +978,10 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
             hipLaunchKernelGGL((k_shade<MULTI, true>), dim3(shade_grid), dim3(kBlock), 0, st, a, A, B, w, cntA, cntB);
         else
             hipLaunchKernelGGL((k_shade<MULTI, false>), dim3(shade_grid), dim3(kBlock), 0, st, a, A, B, w, cntA, cntB);
-        hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, st, cntA, w.cnt + 8);
+        // regeneration: the survivors' buffer is topped up with new camera paths
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, cntA, cntB, w.cnt, (uint32_t)cap, total, ctx->d_status,
+                           a.job_clock);
+        hipLaunchKernelGGL((k_refill<MULTI>), dim3(refill_grid), dim3(kBlock), 0, st, a, B, w);
         return check_hip(ctx, hipGetLastError(), "bounce launch");
     };
     // Graph mode: two bounces (one per buffer parity) captured once and
@@ -968,19 +1013,24 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
             return s;
         }
     }
-    const uint32_t check = graph_on ? std::max<uint32_t>(2, (tn.check_every + 1) & ~1u) : tn.check_every;
-    // With a small depth cap every bounce is issued without reading the live
-    // count (a bounce with no live path costs a few empty launches); without
-    // one the count is read every `check` bounces.
-    const bool poll = !(cap_depth >= 0 && cap_depth <= 32);
-    for (uint64_t done = 0; done < total; done += cap) {
-        const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - done);
-        hipLaunchKernelGGL((k_generate<MULTI>), dim3(blocks_for(n)), dim3(kBlock), 0, st, a, buf[0], w, done, n);
+    // The host stays at most `lag` bounces ahead of the device and stops once
+    // a plan reports every path generated and none live; the bounces issued
+    // after that one find empty queues (a few empty launches each).
+    const uint32_t lag = std::max<uint32_t>(2, graph_on ? (tn.check_every + 1) & ~1u : tn.check_every);
+    if (!MULTI && a.job.max_depth == 0) {  // every path finishes at generation
+        for (uint64_t done = 0; done < total && s == SPT_OK; done += cap) {
+            const uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - done);
+            hipLaunchKernelGGL((k_generate<MULTI>), dim3(blocks_for(n)), dim3(kBlock), 0, st, a, buf[0], w, done, n);
+            s = check_hip(ctx, hipGetLastError(), "generate launch");
+        }
+    } else {
+        ctx->h_status[0] = 0;
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, nullptr, w.cnt, w.cnt, (uint32_t)cap, total, ctx->d_status,
+                           a.job_clock);
+        hipLaunchKernelGGL((k_refill<MULTI>), dim3(refill_grid), dim3(kBlock), 0, st, a, buf[0], w);
         s = check_hip(ctx, hipGetLastError(), "generate launch");
-        if (s != SPT_OK) break;
-        if (!MULTI && a.job.max_depth == 0) continue;
         uint32_t b = 0;
-        while (true) {
+        while (s == SPT_OK) {
             if (graph_on) {
                 s = check_hip(ctx, hipGraphLaunch(exec, st), "graph launch");
                 b += 2;
@@ -988,15 +1038,23 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
                 s = bounce(b);
                 ++b;
             }
-            if (s != SPT_OK) break;
-            if (cap_depth >= 0 && b >= (uint32_t)cap_depth) break;
-            if (poll && b % check == 0) {
-                uint32_t live = 0;
-                s = read_small(ctx, &live, w.cnt + (b & 1), 4, "bounce");
-                if (s != SPT_OK || live == 0) break;
+            if (s != SPT_OK || b <= lag) continue;
+            // plan number 1 is the initial one; bounce j's plan is number j + 2
+            const uint32_t want = b - lag + 1;
+            bool idle = false;
+            while (ctx->h_status[0] < want && !idle) {
+                const hipError_t q = hipStreamQuery(st);
+                if (q == hipSuccess) idle = true;  // drained: the status below is final
+                else if (q != hipErrorNotReady) s = check_hip(ctx, q, "bounce");
+                if (s != SPT_OK) break;
+                __builtin_ia32_pause();
+            }
+            if (s == SPT_OK && ctx->h_status[2] && ctx->h_status[1] == 0) break;
+            if (idle && ctx->h_status[0] < want) {
+                s = set_error(ctx, SPT_ERR_DEVICE, "bounce status did not advance");
+                break;
             }
         }
-        if (s != SPT_OK) break;
     }
     if (s != SPT_OK) {
         (void)stream_wait(ctx, "render abort");
