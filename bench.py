@@ -103,8 +103,8 @@ def _rmse(img, ref):
 
 def guide_experiment(args):
     """Time to equal RMSE on the slit-lit scene: the plain tracer renders
-    1024 spp (timed); the guided tracer renders warm-up passes that only
-    train the guide, then counted passes (training continues after each)
+    1024 spp in one call (timed); the guided tracer renders warm-up passes
+    that only train the guide, then counted passes (training continues after each)
     until its running image reaches the plain image's RMSE against an
     FP64-film reference of oracle_spp samples (independent seed). All render
     and training time is counted, warm-up included; RMSE evaluation between
@@ -157,10 +157,10 @@ def guide_experiment(args):
     r_plain = _rmse(img_plain, ref)
     log(f"plain 1024 spp: {t_plain * 1e3:.1f} ms, RMSE {r_plain:.5f}")
 
-    # guided: warm-up passes (1 spp each, guide trained after every pass,
-    # images discarded but their time counted), then passes of 4 spp that
-    # keep training; the running image of the post-warm-up passes is compared
-    # with the reference after each pass (comparison time not counted)
+    # guided: warm-up passes (guide trained after every pass, images
+    # discarded but their time counted), then counted passes that keep
+    # training; the running image of the counted passes is compared with the
+    # reference after each pass (comparison time not counted)
     dev.guide_init(seed=1, lr=args.guide_lr, batch=262144, steps_per_pass=args.guide_steps, flags=args.guide_flags)
     t_guided, t_train, hit = 0.0, 0.0, None
     curve = []
@@ -185,7 +185,7 @@ def guide_experiment(args):
     warm = dev.alloc(npix * 3 * 4)
     warm.zero()
     for _ in range(args.guide_warmup):
-        guided_pass(1, warm)
+        guided_pass(args.guide_warmup_spp, warm)
     t_warm = t_guided
     # counted passes run back to back; after each, the running film is copied
     # on the device and its RMSE is evaluated only after the run, so the
@@ -286,9 +286,10 @@ def main():
     ap.add_argument("--guide-res", type=int, nargs=2, default=(800, 600))
     ap.add_argument("--oracle-spp", type=int, default=8192, help="spp of the FP64-film reference image")
     ap.add_argument("--guide-lr", type=float, default=1e-2)
-    ap.add_argument("--guide-warmup", type=int, default=16, help="1-spp training passes before the counted image")
-    ap.add_argument("--guide-pass-spp", type=int, default=16, help="spp of each counted guided pass (a square)")
-    ap.add_argument("--guide-steps", type=int, default=8, help="Adam steps per training pass")
+    ap.add_argument("--guide-warmup", type=int, default=4, help="training passes before the counted image")
+    ap.add_argument("--guide-warmup-spp", type=int, default=4, help="spp of each warm-up training pass")
+    ap.add_argument("--guide-pass-spp", type=int, default=100, help="spp of each counted guided pass (a square)")
+    ap.add_argument("--guide-steps", type=int, default=24, help="Adam steps per training pass")
     ap.add_argument("--guide-flags", type=int, default=0,
                     help="SPT_GUIDE_* options: 1 guide layers under a smooth coat, 2 radiance x BSDF-lobe product")
     ap.add_argument("--copy-probe", action="store_true",
