@@ -385,9 +385,11 @@ def main():
     # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_pt_mesh_in_box.json")
-    if os.path.exists(tf) and args.rings == 512 and args.segments == 1024 and sps == 16:
+    if os.path.exists(tf) and args.rings == 512 and args.segments == 1024:
         with open(tf) as fh:
-            traffic = json.load(fh)["hbm_bytes_per_launch"]
+            tj = json.load(fh)
+        if tj.get("spp_per_step") == sps:  # measured on this workload's launches
+            traffic = tj["hbm_bytes_per_launch"]
 
     extra = {}
     if not args.no_closest_hit:
