@@ -25,16 +25,15 @@ constexpr uint32_t kScanThreads = 1024;
 // Path state of one buffer (structure of arrays, one slot per live path).
 // Two buffers alternate: shade reads buffer A and appends survivors to B.
 struct PathBuf {
-    float4* o;       // ray origin
-    float4* d;       // ray direction
-    float4* thr;     // throughput
-    float4* rad;     // radiance so far
-    uint4* meta;     // pixel index, draws consumed, depth, unused
+    float4* o;       // ray origin; .w: pixel index (bits)
+    float4* d;       // ray direction; .w: scene of the path in a multi-scene call (bits)
+    float4* thr;     // throughput; .w: draws consumed (bits)
+    float4* rad;     // radiance so far; .w: depth (bits)
     uint64_t* key;   // RNG key (counter mode) or replay-stream row
     uint64_t* local; // path index within this render call (per-path output)
     uint4* grec;     // guide training records of this path: 3 slots + count
 };
-constexpr size_t kBufBytes = 16 * 6 + 8 * 2;
+constexpr size_t kBufBytes = 16 * 5 + 8 * 2;
 constexpr uint32_t kNoRec = 0xFFFFFFFFu;
 
 // Per-bounce scratch shared by both buffers.
@@ -51,7 +50,7 @@ constexpr size_t kWorkBytes = 16 + 4 + 4;
 struct RenderArgs {
     DevScene sc;
     Job job;                    // single-scene call
-    const Job* jobs;            // multi-scene call: per-job parameters (path meta.w = job)
+    const Job* jobs;            // multi-scene call: per-job parameters (path d.w = job)
     const BatchTile* tiles;     // multi-scene call: generation order
     const uint64_t* tile_start; // first path of each tile (n_tiles + 1 entries)
     uint32_t n_tiles;
@@ -188,11 +187,10 @@ __device__ inline void gen_path(const RenderArgs& a, PathBuf& A, const Work& w, 
         finish_path<MULTI>(a, jb, jid, pix, local, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         return;
     }
-    A.o[s] = make_float4(o.x, o.y, o.z, 0.0f);
-    A.d[s] = make_float4(d.x, d.y, d.z, 0.0f);
-    A.thr[s] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    A.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    A.meta[s] = make_uint4(pix, r.k, 0u, jid);
+    A.o[s] = make_float4(o.x, o.y, o.z, __uint_as_float(pix));
+    A.d[s] = make_float4(d.x, d.y, d.z, __uint_as_float(jid));
+    A.thr[s] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(r.k));
+    A.rad[s] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     A.key[s] = key;
     A.local[s] = local;
     if (a.train) A.grec[s] = make_uint4(kNoRec, kNoRec, kNoRec, 0u);
@@ -286,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, PathBuf A, Work
             o = V3{o4.x, o4.y, o4.z};
             d = V3{d4.x, d4.y, d4.z};
             if (MULTI) {
-                const Job& jb = a.jobs[A.meta[i].w];
+                const Job& jb = a.jobs[__float_as_uint(d4.w)];
                 root = Root{V3{jb.rmin[0], jb.rmin[1], jb.rmin[2]}, V3{jb.rmax[0], jb.rmax[1], jb.rmax[2]},
                             jb.root_leaf, jb.node0, jb.tri0};
             }
@@ -604,9 +602,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const float4 hv = w.hit[i];
         float4 rad = A.rad[i];
         const float4 thr = A.thr[i];
-        const uint4 meta = A.meta[i];
-        const Job& jb = job_of<MULTI>(a, meta.w);
-        const uint32_t depth = meta.z;
+        const float4 o4 = A.o[i], d4 = A.d[i];
+        const uint32_t pix = __float_as_uint(o4.w), jid = __float_as_uint(d4.w);
+        const Job& jb = job_of<MULTI>(a, jid);
+        const uint32_t depth = __float_as_uint(rad.w);
         const uint32_t idx = __float_as_uint(hv.y);
         if (idx == kNoHit) {  // miss: background at depth 0, else 0 (pathtracer.cpp:21-26)
             if (depth == 0) {
@@ -615,10 +614,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 rad.z += thr.z * jb.bg[2];
             }
             if (GUIDED && a.train) finish_records(a, A.grec[i], rad);
-            finish_path<MULTI>(a, jb, meta.w, meta.x, A.local[i], rad);
+            finish_path<MULTI>(a, jb, jid, pix, A.local[i], rad);
             continue;
         }
-        const float4 o4 = A.o[i], d4 = A.d[i];
         const V3 o{o4.x, o4.y, o4.z}, d{d4.x, d4.y, d4.z};
         Hit h;
         h.t = hv.x;
@@ -632,7 +630,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const Frame fr = frame_from_up(nrm);          // pathtracer.cpp:28
         const V3 wo = normalize(to_local(fr, -d));    // pathtracer.cpp:30
         const uint64_t key = A.key[i];
-        Draws r = make_draws(jb, key, meta.y);
+        Draws r = make_draws(jb, key, __float_as_uint(thr.w));
         V3 bsdf, refl, wi;
         float pdf;
         const uint32_t gmode = GUIDED ? guide_mode(a, mid) : 0u;
@@ -713,7 +711,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         if (alive && (jb.max_depth >= 0) && ((int32_t)nd >= jb.max_depth)) alive = false;  // pathtracer.cpp:18
         if (!alive) {
             if (GUIDED && a.train) finish_records(a, A.grec[i], rad);
-            finish_path<MULTI>(a, jb, meta.w, meta.x, A.local[i], rad);
+            finish_path<MULTI>(a, jb, jid, pix, A.local[i], rad);
             continue;
         }
         // pathtracer.cpp:52-54, forwards: throughput *= f * cos / pdf
@@ -742,11 +740,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
         }
         const uint32_t slot = atomicAdd(cntB, 1u);
-        B.o[slot] = make_float4(no.x, no.y, no.z, 0.0f);
-        B.d[slot] = make_float4(wwi.x, wwi.y, wwi.z, 0.0f);
-        B.thr[slot] = make_float4(thr.x * wgt.x, thr.y * wgt.y, thr.z * wgt.z, 0.0f);
-        B.rad[slot] = rad;
-        B.meta[slot] = make_uint4(meta.x, r.k, nd, meta.w);
+        B.o[slot] = make_float4(no.x, no.y, no.z, __uint_as_float(pix));
+        B.d[slot] = make_float4(wwi.x, wwi.y, wwi.z, __uint_as_float(jid));
+        B.thr[slot] = make_float4(thr.x * wgt.x, thr.y * wgt.y, thr.z * wgt.z, __uint_as_float(r.k));
+        B.rad[slot] = make_float4(rad.x, rad.y, rad.z, __uint_as_float(nd));
         B.key[slot] = key;
         B.local[slot] = A.local[i];
         if (GUIDED && a.train) B.grec[slot] = gr;
@@ -865,7 +862,6 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
         b.d = (float4*)take(cap * 16);
         b.thr = (float4*)take(cap * 16);
         b.rad = (float4*)take(cap * 16);
-        b.meta = (uint4*)take(cap * 16);
         b.key = (uint64_t*)take(cap * 8);
         b.local = (uint64_t*)take(cap * 8);
         b.grec = (uint4*)take(cap * 16);
