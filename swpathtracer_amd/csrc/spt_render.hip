@@ -19,7 +19,7 @@ namespace spt {
 namespace {
 
 constexpr uint32_t kMaxSortKeys = 1024;  // materials + 1 (miss) sorted; more keys: unsorted
-constexpr uint32_t kSortBlocks = 256;     // blocks of the histogram / scatter passes
+constexpr uint32_t kMaxSortBlocks = 2048;  // most chunks (blocks) of the histogram / scatter passes
 constexpr uint32_t kScanThreads = 1024;
 
 // Path state of one buffer (structure of arrays, one slot per live path).
@@ -308,11 +308,11 @@ __global__ __launch_bounds__(kBlock) void k_extend(RenderArgs a, PathBuf A, Work
 }
 
 // Stable counting sort of the live slots by material, in three passes over
-// kSortBlocks contiguous chunks: per-chunk histogram (LDS), one exclusive scan
+// contiguous chunks (one per block): per-chunk histogram (LDS), one exclusive scan
 // over [key][chunk], then a scatter that ranks equal keys in slot order. No
 // global atomics: many lanes with one material would serialise on them.
-__device__ inline void chunk_range(uint32_t n, uint32_t b, uint32_t& c0, uint32_t& c1) {
-    const uint32_t per = (n + kSortBlocks - 1) / kSortBlocks;
+__device__ inline void chunk_range(uint32_t n, uint32_t b, uint32_t nb, uint32_t& c0, uint32_t& c1) {
+    const uint32_t per = (n + nb - 1) / nb;
     c0 = min(n, b * per);
     c1 = min(n, c0 + per);
 }
@@ -323,15 +323,15 @@ __global__ __launch_bounds__(kBlock) void k_hist(RenderArgs a, Work w, const uin
     for (uint32_t k = threadIdx.x; k < a.keys; k += kBlock) hist[k] = 0;
     __syncthreads();
     uint32_t c0, c1;
-    chunk_range(n, blockIdx.x, c0, c1);
+    chunk_range(n, blockIdx.x, gridDim.x, c0, c1);
     for (uint32_t e = c0 + threadIdx.x; e < c1; e += kBlock) atomicAdd(&hist[w.skey[e]], 1u);
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < a.keys; k += kBlock) w.bhist[k * kSortBlocks + blockIdx.x] = hist[k];
+    for (uint32_t k = threadIdx.x; k < a.keys; k += kBlock) w.bhist[k * gridDim.x + blockIdx.x] = hist[k];
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan(RenderArgs a, Work w) {
+__global__ __launch_bounds__(kScanThreads) void k_scan(RenderArgs a, Work w, uint32_t nb) {
     __shared__ uint32_t part[kScanThreads];
-    const uint32_t total = a.keys * kSortBlocks;
+    const uint32_t total = a.keys * nb;
     const uint32_t per = (total + kScanThreads - 1) / kScanThreads;
     const uint32_t b0 = min(total, threadIdx.x * per), b1 = min(total, b0 + per);
     uint32_t sum = 0;
@@ -358,12 +358,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RenderArgs a, Work w, const 
     const uint32_t n = *cntA;
     const uint32_t wave = threadIdx.x / 64;
     for (uint32_t k = threadIdx.x; k < a.keys; k += kBlock) {
-        base[k] = w.bhist[k * kSortBlocks + blockIdx.x];
+        base[k] = w.bhist[k * gridDim.x + blockIdx.x];
         for (uint32_t v = 0; v < kBlock / 64; ++v) wcnt[v][k] = 0;
     }
     __syncthreads();
     uint32_t c0, c1;
-    chunk_range(n, blockIdx.x, c0, c1);
+    chunk_range(n, blockIdx.x, gridDim.x, c0, c1);
     const uint64_t lt = __lanemask_lt();
     for (uint32_t r0 = c0; r0 < c1; r0 += kBlock) {
         const uint32_t e = r0 + threadIdx.x;
@@ -839,7 +839,7 @@ template <bool MULTI>
 spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_depth, const spt_tuning& tn,
                      spt_render_stats& stats) {
     const uint64_t cap = std::min<uint64_t>(total, tn.path_capacity);
-    const size_t bytes = (size_t)cap * (2 * kBufBytes + kWorkBytes + 4 * 8) + (size_t)kMaxSortKeys * kSortBlocks * 4 +
+    const size_t bytes = (size_t)cap * (2 * kBufBytes + kWorkBytes + 4 * 8) + (size_t)kMaxSortKeys * kMaxSortBlocks * 4 +
                          4096;
     if (bytes > ctx->paths_bytes) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -870,7 +870,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
     w.hit = (float4*)take(cap * 16);
     w.skey = (uint32_t*)take(cap * 4);
     w.perm = (uint32_t*)take(cap * 4);
-    w.bhist = (uint32_t*)take((size_t)kMaxSortKeys * kSortBlocks * 4);
+    w.bhist = (uint32_t*)take((size_t)kMaxSortKeys * kMaxSortBlocks * 4);
     w.cnt = (uint32_t*)take(256);
     if ((size_t)(base - reinterpret_cast<char*>(ctx->d_paths)) > ctx->paths_bytes)
         return set_error(ctx, SPT_ERR_NOMEM, "path state layout exceeds its allocation");
@@ -899,6 +899,9 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
     const uint32_t ext_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)(per_cu * ctx->num_cus));
     const uint32_t shade_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * tn.shade_blocks_per_cu);
     const uint32_t refill_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * 8);
+    // sort chunks: many (latency hiding) unless many keys would make the scan long
+    const uint32_t sort_blocks =
+        a.keys ? std::max<uint32_t>(256, std::min<uint32_t>(kMaxSortBlocks, 16384 / a.keys)) : 256;
     spt_status s = ensure_spill(ctx, (size_t)ext_grid * kBlock *
                                          (a.sc.stack_cap4 > kLdsStack ? a.sc.stack_cap4 - kLdsStack : 1));
     if (s != SPT_OK) return s;
@@ -955,9 +958,9 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
             ++stats.extend_launches;
         }
         if (a.keys) {
-            hipLaunchKernelGGL(k_hist, dim3(kSortBlocks), dim3(kBlock), 0, st, a, w, cntA);
-            hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, a, w);
-            hipLaunchKernelGGL(k_scatter, dim3(kSortBlocks), dim3(kBlock), 0, st, a, w, cntA);
+            hipLaunchKernelGGL(k_hist, dim3(sort_blocks), dim3(kBlock), 0, st, a, w, cntA);
+            hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, a, w, sort_blocks);
+            hipLaunchKernelGGL(k_scatter, dim3(sort_blocks), dim3(kBlock), 0, st, a, w, cntA);
         }
         if (a.guided) {
             if (timed) {
