@@ -571,7 +571,8 @@ struct Job {
 };
 
 // A run of consecutive paths of one job: pixels [p0, p0 + np) of its
-// viewport (row-major), samples [s0, s0 + ns) of each, pixel-major.
+// viewport (row-major), samples [s0, s0 + ns) of each; the paths run over the
+// pixels for one sample, then the next sample.
 struct BatchTile {
     uint32_t job, p0, np, s0, ns, pad[3];
 };

@@ -144,7 +144,8 @@ __device__ inline Draws make_draws(const Job& jb, uint64_t key, uint32_t k) {
 
 // Camera sample of path g into slot s of buffer A:
 // integrators/pathtracer.cpp:104-113. A single-scene call numbers its paths
-// pixel-major over the viewport; a multi-scene call walks its tile list.
+// sample by sample over the viewport's pixels; a multi-scene call walks its
+// tile list. The per-path output index is pixel * spp_count + sample either way.
 // With max depth 0 (single-scene calls only) the path finishes here.
 template <bool MULTI>
 __device__ inline void gen_path(const RenderArgs& a, PathBuf& A, const Work& w, uint32_t s, uint64_t g) {
@@ -163,13 +164,16 @@ __device__ inline void gen_path(const RenderArgs& a, PathBuf& A, const Work& w, 
         const BatchTile tl = a.tiles[lo];
         const uint64_t q = g - a.tile_start[lo];
         jid = tl.job;
-        pl = tl.p0 + q / tl.ns;
-        smp = tl.s0 + (uint32_t)(q % tl.ns);
+        pl = tl.p0 + q % tl.np;
+        smp = tl.s0 + (uint32_t)(q / tl.np);
         local = pl * a.jobs[jid].spp_count + (smp - a.jobs[jid].spp_begin);
     } else {
-        pl = g / a.job.spp_count;
-        smp = a.job.spp_begin + (uint32_t)(g % a.job.spp_count);
-        local = g;
+        // pixel-minor: consecutive paths are neighbouring pixels (coherent
+        // primary rays, and no pile-up of one pixel's film atomics in a wave)
+        const uint64_t npl = (uint64_t)a.job.vw * a.job.vh;
+        pl = g % npl;
+        smp = a.job.spp_begin + (uint32_t)(g / npl);
+        local = pl * a.job.spp_count + (smp - a.job.spp_begin);
     }
     const Job& jb = job_of<MULTI>(a, jid);
     const uint32_t x = jb.vl + (uint32_t)(pl % jb.vw), y = jb.vt + (uint32_t)(pl / jb.vw);
