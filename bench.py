@@ -269,10 +269,10 @@ def batch_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp-per-step", type=int, default=512,
-                    help="samples per pixel of one step (one render call); 2 steps = the 1024-spp workload")
+    ap.add_argument("--spp-per-step", type=int, default=1024,
+                    help="samples per pixel of one step (one render call); 1 step = the 1024-spp image")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-closest-hit", action="store_true")
     ap.add_argument("--no-sort", action="store_true", help="skip the material sort between stages")

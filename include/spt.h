@@ -334,8 +334,8 @@ spt_status spt_event_elapsed(spt_ctx* ctx, int a, int b, float* ms);
 /* Queue and tile sizes of the wavefront pipeline. A zero field means the
  * default. Results never depend on these, only speed. */
 typedef struct spt_tuning {
-    uint32_t path_capacity;        /* paths in flight (queue size, at most 2^26; default 67,108,864:
-                                      about 280 B of device memory per slot, 440 B when guided;
+    uint32_t path_capacity;        /* paths in flight (queue size, at most 2^27; default 134,217,728:
+                                      about 250 B of device memory per slot, 410 B when guided;
                                       a render allocates min(paths of the call, capacity)) */
     uint32_t extend_blocks_per_cu; /* persistent traversal grid (default: occupancy limit) */
     uint32_t shade_blocks_per_cu;  /* shading grid (default 8) */

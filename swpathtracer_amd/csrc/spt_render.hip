@@ -766,7 +766,7 @@ size_t guide_paths_lds_bytes() {
 // Defaults for the zero fields of a tuning.
 spt_tuning resolve_tuning(const spt_tuning& t) {
     spt_tuning r = t;
-    if (r.path_capacity == 0) r.path_capacity = 1u << 26;
+    if (r.path_capacity == 0) r.path_capacity = 1u << 27;
     if (r.shade_blocks_per_cu == 0) r.shade_blocks_per_cu = 8;
     if (r.use_graph == 0) r.use_graph = 2;
     if (r.tile_pixels == 0) r.tile_pixels = 4096;

@@ -90,7 +90,7 @@ def test_autotune_single_scene(device):
     sc = scenes.cornell_icosphere(subdiv=3, width=64, height=64, spp=16)
     _, ref = _single(device, sc)
     t = device.autotune(sc.camera, sc.integrator, sample_paths=1 << 16)
-    assert t["path_capacity"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
+    assert t["path_capacity"] in (1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27)
     assert t["use_graph"] in (1, 2)
     assert device.get_tuning() == t
     _, after = device.render_image(sc.camera, sc.integrator, per_path=True)
