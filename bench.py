@@ -161,7 +161,8 @@ def guide_experiment(args):
     # discarded but their time counted), then counted passes that keep
     # training; the running image of the counted passes is compared with the
     # reference after each pass (comparison time not counted)
-    dev.guide_init(seed=1, lr=args.guide_lr, batch=262144, steps_per_pass=args.guide_steps, flags=args.guide_flags)
+    dev.guide_init(seed=1, lr=args.guide_lr, batch=262144, steps_per_pass=args.guide_steps, flags=args.guide_flags,
+                   train_every=args.guide_train_every)
     t_guided, t_train, hit = 0.0, 0.0, None
     curve = []
     k = 0
@@ -290,6 +291,8 @@ def main():
     ap.add_argument("--guide-warmup-spp", type=int, default=4, help="spp of each warm-up training pass")
     ap.add_argument("--guide-pass-spp", type=int, default=100, help="spp of each counted guided pass (a square)")
     ap.add_argument("--guide-steps", type=int, default=24, help="Adam steps per training pass")
+    ap.add_argument("--guide-train-every", type=int, default=0,
+                    help="> 0: train inside each guided call every this many bounces")
     ap.add_argument("--guide-flags", type=int, default=0,
                     help="SPT_GUIDE_* options: 1 guide layers under a smooth coat, 2 radiance x BSDF-lobe product")
     ap.add_argument("--copy-probe", action="store_true",

@@ -433,6 +433,11 @@ typedef struct spt_guide_config {
                                 records; spt_guide_train_collected takes one Adam step
                                 per batch of them (default 1) */
     uint32_t flags;        /* SPT_GUIDE_* options (default 0) */
+    uint32_t train_every;  /* > 0: a GUIDE_TRAIN render also trains inside the call, one
+                              round every train_every bounces (Adam steps on the records of
+                              paths recorded at least one round earlier; the record buffer
+                              is a ring of batch x steps_per_pass records); 0: records are
+                              kept for spt_guide_train_collected (default 0) */
 } spt_guide_config;
 
 /* Guide options. COATED: also guide a Lambertian / GGX layer under one smooth

@@ -99,7 +99,7 @@ class RenderStats(C.Structure):
 class GuideConfig(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("learning_rate", C.c_float), ("loss_scale", C.c_float),
                 ("bsdf_fraction", C.c_float), ("batch", C.c_uint32), ("steps_per_pass", C.c_uint32),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("train_every", C.c_uint32)]
 
 
 class GuideStats(C.Structure):

@@ -402,9 +402,9 @@ class Device:
 
     # learned guide -----------------------------------------------------
     def guide_init(self, seed: int = 1, lr: float = 5e-3, loss_scale: float = 65536.0, bsdf_fraction: float = 0.5,
-                   batch: int = 262144, steps_per_pass: int = 1, flags: int = 0):
+                   batch: int = 262144, steps_per_pass: int = 1, flags: int = 0, train_every: int = 0):
         cfg = L.GuideConfig(seed=seed, learning_rate=lr, loss_scale=loss_scale, bsdf_fraction=bsdf_fraction,
-                            batch=batch, steps_per_pass=steps_per_pass, flags=flags)
+                            batch=batch, steps_per_pass=steps_per_pass, flags=flags, train_every=train_every)
         self._check(self.lib.spt_guide_init(self.h, C.byref(cfg)))
 
     def guide_set_params(self, theta: np.ndarray):

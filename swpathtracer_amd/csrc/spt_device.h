@@ -633,6 +633,8 @@ struct GuideState {
     uint64_t collected = 0;
     float lr = 5e-3f, loss_scale = 65536.0f, bsdf_fraction = 0.5f;  // loss_scale: the configured start
     uint32_t flags = 0;  // SPT_GUIDE_*
+    uint32_t train_every = 0;      // in-call training rounds every this many bounces (0: none)
+    double rays_per_path = 4.0;    // extension rays per path of the last training render
     uint32_t batch = 262144;
     double train_ms = 0.0;
     float* d_gout = nullptr;  // per-path raw outputs during a guided render
@@ -766,6 +768,9 @@ spt_status launch_raycast(spt_ctx* ctx, const spt_camera* cam, uint32_t kind, vo
 spt_status launch_camera_rays(spt_ctx* ctx, const spt_camera* cam, spt_ray32* d_rays);
 GuideBox guide_box(const spt_ctx* ctx);
 spt_status guide_eval_launch(spt_ctx* ctx, const float* d_q, uint32_t stride, uint32_t n, float* d_out);
+// One Adam step on n records (queued on the context stream; loss_out, when
+// given, waits for the step).
+spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, float* loss_out);
 spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* p, float* d_rgb32,
                          double* d_rgb64, float* d_path_rgb);
 spt_tuning resolve_tuning(const spt_tuning& t);

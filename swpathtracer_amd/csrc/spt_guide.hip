@@ -613,6 +613,7 @@ spt_status spt_guide_init(spt_ctx* ctx, const spt_guide_config* cfg) {
     gs.loss_scale = c.loss_scale;
     gs.bsdf_fraction = c.bsdf_fraction;
     gs.flags = c.flags;
+    gs.train_every = c.train_every;
     gs.batch = c.batch;
     gs.train_ms = 0.0;
     gs.kernel_ms = 0.0;

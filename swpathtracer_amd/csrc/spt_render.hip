@@ -69,16 +69,25 @@ struct RenderArgs {
     uint32_t* rec_count;  // [0]: records written, [1]: candidate vertices
     uint32_t rec_cap;
     uint32_t rec_stride;  // record 1 in rec_stride candidates (hash of key and depth)
+    uint32_t rec_ring;    // in-call training: record slots wrap around rec_cap
 };
 
 __device__ inline float lum(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
+// Tag of record k of the path with RNG key `key`, kept in the record: with a
+// ring of record slots a slot can be reused while its first path is alive,
+// and that path must then leave the new record alone.
+__device__ inline uint32_t rec_tag(uint64_t key, uint32_t k) {
+    return (uint32_t)mix64(key * 3u + k + 1u);
+}
+
 // Training targets of a finished path: the radiance that arrived along each
 // recorded wi is (L - radiance before it) / throughput after it.
-__device__ inline void finish_records(const RenderArgs& a, uint4 gr, float4 L) {
+__device__ inline void finish_records(const RenderArgs& a, uint4 gr, float4 L, uint64_t key) {
     const uint32_t sl[3] = {gr.x, gr.y, gr.z};
     for (uint32_t k = 0; k < gr.w && k < 3; ++k) {
         float* r = a.records + (size_t)sl[k] * guide::kRecord;
+        if (__float_as_uint(r[20]) != rec_tag(key, k)) continue;  // slot reused
         const float lt = lum(V3{r[17], r[18], r[19]});
         const float li = lum(V3{L.x - r[14], L.y - r[15], L.z - r[16]});
         r[12] = (lt > 0.0f && li > 0.0f) ? r[13] * li / lt : 0.0f;
@@ -244,6 +253,7 @@ __global__ void k_plan(uint32_t* cntA, uint32_t* cntB, uint32_t* cnt, uint32_t c
         volatile uint32_t* hs = status;
         hs[1] = have + n;
         hs[2] = (g0 + n == total) ? 1u : 0u;
+        hs[3] = cnt[16];  // guide records written so far
         __threadfence_system();
         hs[0] = seq;
     }
@@ -617,7 +627,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 rad.y += thr.y * jb.bg[1];
                 rad.z += thr.z * jb.bg[2];
             }
-            if (GUIDED && a.train) finish_records(a, A.grec[i], rad);
+            if (GUIDED && a.train) finish_records(a, A.grec[i], rad, A.key[i]);
             finish_path<MULTI>(a, jb, jid, pix, A.local[i], rad);
             continue;
         }
@@ -714,7 +724,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const uint32_t nd = depth + 1;
         if (alive && (jb.max_depth >= 0) && ((int32_t)nd >= jb.max_depth)) alive = false;  // pathtracer.cpp:18
         if (!alive) {
-            if (GUIDED && a.train) finish_records(a, A.grec[i], rad);
+            if (GUIDED && a.train) finish_records(a, A.grec[i], rad, A.key[i]);
             finish_path<MULTI>(a, jb, jid, pix, A.local[i], rad);
             continue;
         }
@@ -729,9 +739,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         if (cand && (a.rec_stride <= 1u ||
                      mix64(key + (uint64_t)(depth + 1) * 0x9E3779B97F4A7C15ull) % a.rec_stride == 0u)) {
-            const uint32_t rs = atomicAdd(a.rec_count, 1u);
+            uint32_t rs = atomicAdd(a.rec_count, 1u);
+            if (a.rec_ring) rs %= a.rec_cap;
             if (rs < a.rec_cap) {
                 float* rec = a.records + (size_t)rs * guide::kRecord;
+                rec[20] = __uint_as_float(rec_tag(key, gr.w));
                 const V3 wdir = to_world(fr, wi);
                 const float vals[20] = {pos.x, pos.y, pos.z, nrm.x, nrm.y, nrm.z, -d.x, -d.y, -d.z, wdir.x, wdir.y,
                                         wdir.z, 0.0f, rec_w, rad.x, rad.y, rad.z,
@@ -915,7 +927,8 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
     s = check_hip(ctx, hipMemsetAsync(w.cnt, 0, 256, st), "counters");
     if (s != SPT_OK) return s;
     if (a.train) {  // records continue after those collected since the last training step
-        s = check_hip(ctx, hipMemsetD32Async((hipDeviceptr_t)(w.cnt + 16), (int)ctx->guide.collected, 1, st),
+        s = check_hip(ctx,
+                      hipMemsetD32Async((hipDeviceptr_t)(w.cnt + 16), a.rec_ring ? 0 : (int)ctx->guide.collected, 1, st),
                       "record counter");
         if (s != SPT_OK) return s;
     }
@@ -1035,6 +1048,9 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
         hipLaunchKernelGGL((k_refill<MULTI>), dim3(refill_grid), dim3(kBlock), 0, st, a, buf[0], w);
         s = check_hip(ctx, hipGetLastError(), "generate launch");
         uint32_t b = 0;
+        // in-call training (ring of records): train aligned batches of the
+        // records counted one round earlier; skip what is about to be reused
+        uint32_t next_train = ctx->guide.train_every, prev_count = 0, trained = 0;
         while (s == SPT_OK) {
             if (graph_on) {
                 s = check_hip(ctx, hipGraphLaunch(exec, st), "graph launch");
@@ -1059,6 +1075,17 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
                 s = set_error(ctx, SPT_ERR_DEVICE, "bounce status did not advance");
                 break;
             }
+            if (a.rec_ring && s == SPT_OK && b >= next_train) {
+                const uint32_t now = ctx->h_status[3], upto = prev_count, bt = ctx->guide.batch;
+                prev_count = now;
+                next_train = b + ctx->guide.train_every;
+                if (now > a.rec_cap / 2) trained = std::max(trained, (now - a.rec_cap / 2 + bt - 1) / bt * bt);
+                while (s == SPT_OK && trained + bt <= upto) {
+                    s = guide_train_launch(ctx, ctx->guide.d_records + (size_t)(trained % a.rec_cap) * guide::kRecord, bt,
+                                           nullptr);
+                    trained += bt;
+                }
+            }
         }
     }
     if (s != SPT_OK) {
@@ -1073,7 +1100,9 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
     stats.stream_overflows = cnt[2];
     if (a.train) {
         GuideState& gs = ctx->guide;
-        gs.collected = std::min<uint64_t>(cnt[16], a.rec_cap);
+        // a ring of records was trained inside the call; otherwise the
+        // records wait for spt_guide_train_collected
+        gs.collected = a.rec_ring ? 0 : std::min<uint64_t>(cnt[16], a.rec_cap);
         gs.candidates = cnt[17];
         // next pass: record about one batch worth of candidates
         gs.cand_per_path = total ? (double)cnt[17] / (double)total : 0.0;
@@ -1084,6 +1113,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
     }
     for (hipEvent_t e : gev) (void)hipEventDestroy(e);
     stats.rays = (uint64_t)cnt[10] | ((uint64_t)cnt[11] << 32);
+    if (a.train && total) ctx->guide.rays_per_path = std::max(1.0, (double)stats.rays / (double)total);
     stats.bounces = (uint64_t)cnt[12] | ((uint64_t)cnt[13] << 32);
     if (timed) {
         for (uint64_t i = 0; i < stats.extend_launches; ++i) {
@@ -1136,10 +1166,20 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     a.records = ctx->guide.d_records;
     a.rec_cap = ctx->guide.records_cap;
     // record about records_cap of this render's candidate vertices, estimated
-    // from the candidates per path of the previous training render
-    if (ctx->guide.cand_per_path > 0.0 && ctx->guide.records_cap > 0)
-        ctx->guide.rec_stride = (uint32_t)std::min<double>(
-            1 << 30, std::max(1.0, std::ceil(ctx->guide.cand_per_path * (double)total / ctx->guide.records_cap)));
+    // from the candidates per path of the previous training render; with
+    // in-call training about a quarter of the ring per training round (a
+    // bounce of a full queue meets about capacity / rays-per-path paths)
+    a.rec_ring = (a.train && ctx->guide.train_every) ? 1u : 0u;
+    if (ctx->guide.cand_per_path > 0.0 && ctx->guide.records_cap > 0) {
+        double want = ctx->guide.cand_per_path * (double)total;
+        if (a.rec_ring) {
+            const double per_bounce = ctx->guide.cand_per_path / ctx->guide.rays_per_path *
+                                      (double)std::min<uint64_t>(total, resolve_tuning(ctx->tuning).path_capacity);
+            want = per_bounce * ctx->guide.train_every * 4.0;
+        }
+        ctx->guide.rec_stride =
+            (uint32_t)std::min<double>(1 << 30, std::max(1.0, std::ceil(want / ctx->guide.records_cap)));
+    }
     a.rec_stride = ctx->guide.rec_stride;
     const uint32_t keys = ctx->n_mats + 1;
     a.keys = (!(prm->flags & SPT_RENDER_NO_SORT) && keys <= kMaxSortKeys) ? keys : 0u;

@@ -290,3 +290,28 @@ def test_guided_training_render_collects_records(device, flags):
     st = device.guide_stats()
     assert st["steps"] + st["skipped_steps"] == 1 and st["records"] == 0
     assert np.all(np.isfinite(img))
+
+
+def test_in_call_training_trains_during_the_render(device):
+    """train_every > 0: Adam steps run between bounces of one render call on
+    a ring of records (tags keep a reused slot from being finished by the
+    path that held it before); nothing is left for train_collected."""
+    sc = scenes.slit_box(rings=32, segments=64, width=96, height=72, spp=64)
+    bvh = api.Bvh(sc.verts)
+    device.set_scene(bvh, sc.normals, sc.mat_ids)
+    device.set_materials(sc.materials)
+    device.guide_init(seed=1, batch=4096, steps_per_pass=8, train_every=2)
+    flags = L.SPT_RENDER_GUIDED | L.SPT_RENDER_GUIDE_TRAIN
+    device.set_tuning(path_capacity=65536)  # many bounces per call: several training rounds
+    try:
+        for _ in range(2):  # the first call sets the record stride
+            img, _ = device.render_image(sc.camera, sc.integrator, extra_flags=flags)
+    finally:
+        device.set_tuning(path_capacity=0)
+    st = device.guide_stats()
+    print("steps", st["steps"], "skipped", st["skipped_steps"], "loss", st["last_loss"], "stride", st["record_stride"])
+    assert st["steps"] + st["skipped_steps"] >= 4
+    assert st["records"] == 0
+    assert np.isfinite(st["last_loss"])
+    assert np.all(np.isfinite(img))
+    assert device.guide_train_collected() == 0.0
