@@ -336,7 +336,8 @@ spt_status spt_event_elapsed(spt_ctx* ctx, int a, int b, float* ms);
 typedef struct spt_tuning {
     uint32_t path_capacity;        /* paths in flight (queue size, at most 2^27; default 134,217,728:
                                       about 250 B of device memory per slot, 410 B when guided;
-                                      a render allocates min(paths of the call, capacity)) */
+                                      a render allocates min(paths of the call, capacity), halved
+                                      while the device has no room, down to 2^20) */
     uint32_t extend_blocks_per_cu; /* persistent traversal grid (default: occupancy limit) */
     uint32_t shade_blocks_per_cu;  /* shading grid (default 8) */
     uint32_t use_graph;            /* 1: bounce loop captured once in a HIP graph and replayed;

@@ -854,17 +854,26 @@ namespace {
 template <bool MULTI>
 spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_depth, const spt_tuning& tn,
                      spt_render_stats& stats) {
-    const uint64_t cap = std::min<uint64_t>(total, tn.path_capacity);
-    const size_t bytes = (size_t)cap * (2 * kBufBytes + kWorkBytes + 4 * 8) + (size_t)kMaxSortKeys * kMaxSortBlocks * 4 +
-                         4096;
-    if (bytes > ctx->paths_bytes) {
+    uint64_t cap = std::min<uint64_t>(total, tn.path_capacity);
+    auto state_bytes = [](uint64_t c) {
+        return (size_t)c * (2 * kBufBytes + kWorkBytes + 4 * 8) + (size_t)kMaxSortKeys * kMaxSortBlocks * 4 + 4096;
+    };
+    if (state_bytes(cap) > ctx->paths_bytes) {
         (void)hipStreamSynchronize(ctx->stream);
         if (ctx->d_paths) (void)hipFree(ctx->d_paths);
         ctx->d_paths = nullptr;
         ctx->paths_bytes = 0;
-        spt_status s = check_hip(ctx, hipMalloc(&ctx->d_paths, bytes), "path state");
-        if (s != SPT_OK) return s;
-        ctx->paths_bytes = bytes;
+        // a device without room for the queue gets a smaller one (slower,
+        // same results), down to 2^20 paths
+        while (true) {
+            const hipError_t e = hipMalloc(&ctx->d_paths, state_bytes(cap));
+            if (e == hipSuccess) break;
+            (void)hipGetLastError();  // the failed allocation is not the next launch's error
+            ctx->d_paths = nullptr;
+            if (e != hipErrorOutOfMemory || cap <= (1u << 20)) return check_hip(ctx, e, "path state");
+            cap = std::max<uint64_t>(cap / 2, 1u << 20);
+        }
+        ctx->paths_bytes = state_bytes(cap);
     }
     char* base = reinterpret_cast<char*>(ctx->d_paths);
     auto take = [&](size_t sz) {
