@@ -18,6 +18,7 @@ namespace spt {
 
 namespace {
 
+constexpr uint32_t kLdsMats = 48;         // shading stages up to this many materials in LDS
 constexpr uint32_t kMaxSortKeys = 1024;  // materials + 1 (miss) sorted; more keys: unsorted
 constexpr uint32_t kMaxSortBlocks = 2048;  // most chunks (blocks) of the histogram / scatter passes
 constexpr uint32_t kScanThreads = 1024;
@@ -609,6 +610,17 @@ __device__ inline void layered_eval_guided(const spt_material& m, V3 wo, const F
 template <bool MULTI, bool GUIDED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(RenderArgs a, PathBuf A, PathBuf B, Work w,
                                                   const uint32_t* __restrict__ cntA, uint32_t* __restrict__ cntB) {
+    // the material table in LDS when it is small (each lane reads its
+    // layered material field by field)
+    __shared__ spt_material smats[kLdsMats];
+    const bool lds_mats = a.sc.n_mats <= kLdsMats;
+    if (lds_mats) {
+        const uint32_t words = a.sc.n_mats * (uint32_t)(sizeof(spt_material) / 4);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.sc.mats);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(smats);
+        for (uint32_t k = threadIdx.x; k < words; k += kBlock) dst[k] = src[k];
+    }
+    __syncthreads();
     const uint32_t n = *cntA;
     const uint32_t gthreads = gridDim.x * kBlock;
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gthreads) {
@@ -640,7 +652,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const V3 pos = o + d * h.t;  // triangle.cpp:69
         const V3 nrm = shading_normal(a.sc, h);
         const uint32_t mid = w.skey[i];
-        const spt_material& m = a.sc.mats[mid];
+        const spt_material& m = lds_mats ? smats[mid] : a.sc.mats[mid];
         const Frame fr = frame_from_up(nrm);          // pathtracer.cpp:28
         const V3 wo = normalize(to_local(fr, -d));    // pathtracer.cpp:30
         const uint64_t key = A.key[i];
