@@ -339,14 +339,15 @@ typedef struct spt_tuning {
                                       a render allocates min(paths of the call, capacity), halved
                                       while the device has no room, down to 2^20) */
     uint32_t extend_blocks_per_cu; /* persistent traversal grid (default: occupancy limit) */
-    uint32_t shade_blocks_per_cu;  /* shading grid (default 8) */
+    uint32_t shade_blocks_per_cu;  /* shading grid (default 16) */
     uint32_t use_graph;            /* 1: bounce loop captured once in a HIP graph and replayed;
                                       2: plain launches (default) */
     uint32_t tile_pixels;          /* multi-scene calls: pixels per generation tile (default 4096) */
     uint32_t tile_samples;         /* multi-scene calls: samples per tile (default 16) */
-    uint32_t check_every;          /* bounces between reads of the live count (default 4) */
+    uint32_t check_every;          /* bounces the host may run ahead of the device (default 4) */
     uint32_t refill_lanes;         /* a traversal wave fetches new rays once this many of
-                                      its 64 lanes are idle (default 16) */
+                                      its 64 lanes are idle (default 48 in path tracing,
+                                      16 in spt_trace) */
 } spt_tuning;
 spt_status spt_set_tuning(spt_ctx* ctx, const spt_tuning* t);
 spt_status spt_get_tuning(spt_ctx* ctx, spt_tuning* t);

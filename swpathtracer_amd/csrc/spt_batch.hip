@@ -77,10 +77,10 @@ spt_status tune(spt_ctx* ctx, bool batch, F run, spt_tuning* out) {
     std::vector<Dim> dims = {
         {&spt_tuning::path_capacity, {1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27}},
         {&spt_tuning::extend_blocks_per_cu, {0u, 6u, 4u, 3u, 2u}},
-        {&spt_tuning::shade_blocks_per_cu, {4u, 8u, 16u}},
+        {&spt_tuning::shade_blocks_per_cu, {8u, 16u, 32u}},
         {&spt_tuning::use_graph, {2u, 1u}},
         {&spt_tuning::check_every, {2u, 4u, 8u}},
-        {&spt_tuning::refill_lanes, {8u, 16u, 32u}},
+        {&spt_tuning::refill_lanes, {16u, 32u, 48u, 56u}},
     };
     if (batch) {
         dims.push_back({&spt_tuning::tile_pixels, {1024u, 4096u, 16384u, 65536u}});

@@ -141,7 +141,10 @@ spt_status launch_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_h
         DevScene sc = ctx->scene();
         const float4* rays = reinterpret_cast<const float4*>(d_rays);
         uint2* hits = reinterpret_cast<uint2*>(d_hits);
-        const uint32_t refill = std::min<uint32_t>(64, std::max<uint32_t>(1, resolve_tuning(ctx->tuning).refill_lanes));
+        // incoherent ray batches prefer eager refills (16 idle lanes); the path
+        // tracer's default (48) is for its extension rays
+        const uint32_t want = ctx->tuning.refill_lanes ? ctx->tuning.refill_lanes : 16u;
+        const uint32_t refill = std::min<uint32_t>(64, std::max<uint32_t>(1, want));
         if (mode == SPT_TRACE_FETCH)
             hipLaunchKernelGGL(k_trace_p<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, (uint32_t)n, hits,
                                d_tests, d_isects, ctx->d_spill, ctx->d_flags, refill);
