@@ -768,9 +768,9 @@ spt_status launch_raycast(spt_ctx* ctx, const spt_camera* cam, uint32_t kind, vo
 spt_status launch_camera_rays(spt_ctx* ctx, const spt_camera* cam, spt_ray32* d_rays);
 GuideBox guide_box(const spt_ctx* ctx);
 spt_status guide_eval_launch(spt_ctx* ctx, const float* d_q, uint32_t stride, uint32_t n, float* d_out);
-// One Adam step on n records (queued on the context stream; loss_out, when
-// given, waits for the step).
-spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, float* loss_out);
+// One Adam step on n records, record i at d_rec + i * rs records (queued on
+// the context stream; loss_out, when given, waits for the step).
+spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, float* loss_out, uint32_t rs);
 spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* p, float* d_rgb32,
                          double* d_rgb64, float* d_path_rgb);
 spt_tuning resolve_tuning(const spt_tuning& t);

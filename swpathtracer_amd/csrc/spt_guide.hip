@@ -48,10 +48,11 @@ __global__ __launch_bounds__(kBlock) void k_guide_eval(const float* __restrict__
 }
 
 // Sum of the record weights (for the batch-mean normalisation).
-__global__ __launch_bounds__(kBlock) void k_guide_wsum(const float* __restrict__ rec, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k_guide_wsum(const float* __restrict__ rec, uint32_t n, uint32_t rs,
                                                        float* __restrict__ acc) {
     float s = 0.0f;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += rec[(size_t)i * kRecord + 12];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        s += rec[(size_t)i * rs * kRecord + 12];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (__lane_id() == 0) atomicAdd(acc, s);
 }
@@ -111,8 +112,9 @@ __device__ inline bf16x4 ld_wT(const unsigned short* W, int K, int ot, int kt) {
 // (rows n..rows16: zero gradient). One block = 256 records, lane = record.
 __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ theta,
                                                       const unsigned short* __restrict__ wbf,
-                                                      const float* __restrict__ rec, uint32_t n, uint32_t n16,
-                                                      GuideCtl* __restrict__ ctl, GuideBox box, TrainScratch ts) {
+                                                      const float* __restrict__ rec, uint32_t n, uint32_t rs,
+                                                      uint32_t n16, GuideCtl* __restrict__ ctl, GuideBox box,
+                                                      TrainScratch ts) {
     extern __shared__ float dyn[];
     SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
     WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
     const uint32_t wbase = blockIdx.x * kBlock + wave * 64;
     float f[kEnc];
     if (s < n) {
-        const float* p = rec + (size_t)s * kRecord;
+        const float* p = rec + (size_t)s * rs * kRecord;
         encode(V3{p[0], p[1], p[2]}, V3{p[3], p[4], p[5]}, V3{p[6], p[7], p[8]}, box.bmin, box.bext, f);
     } else {
         for (int i = 0; i < kEnc; ++i) f[i] = 0.0f;
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
     if (s < n16) {
         float dy[kOutP];
         if (s < n) {
-            const float* p = rec + (size_t)s * kRecord;
+            const float* p = rec + (size_t)s * rs * kRecord;
             loss = loss_grad(ys + l * kOut, V3{p[9], p[10], p[11]}, p[12] / wmean, dy);
         } else {
             for (int i = 0; i < kOutP; ++i) dy[i] = 0.0f;
@@ -446,7 +448,7 @@ spt_status train_scratch(spt_ctx* ctx, uint32_t n16, uint32_t splits, TrainScrat
 
 // Queues the gradient of one step (ctl->wsum and ctl->loss, grad = scaled
 // gradient) on the context stream.
-spt_status queue_gradient(spt_ctx* ctx, const float* d_rec, uint32_t n, hipEvent_t e0, hipEvent_t e1) {
+spt_status queue_gradient(spt_ctx* ctx, const float* d_rec, uint32_t n, uint32_t rs, hipEvent_t e0, hipEvent_t e1) {
     GuideState& gs = ctx->guide;
     hipStream_t st = ctx->stream;
     GuideCtl* ctl = gs.ctl();
@@ -456,10 +458,10 @@ spt_status queue_gradient(spt_ctx* ctx, const float* d_rec, uint32_t n, hipEvent
     TrainScratch ts;
     spt_status s = train_scratch(ctx, n16, splits, ts);
     if (s != SPT_OK) return s;
-    hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, &ctl->wsum);
+    hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(kBlock), 0, st, d_rec, n, rs, &ctl->wsum);
     if (e0) (void)hipEventRecord(e0, st);
     hipLaunchKernelGGL(k_train_fwd, dim3((n16 + kBlock - 1) / kBlock), dim3(kBlock), fwd_lds_bytes(), st, gs.d_theta,
-                       gs.d_wbf, d_rec, n, n16, ctl, guide_box(ctx), ts);
+                       gs.d_wbf, d_rec, n, rs, n16, ctl, guide_box(ctx), ts);
     hipLaunchKernelGGL(k_train_bwd, dim3(std::min<uint32_t>((n16 + 63) / 64, (uint32_t)ctx->num_cus * 4)), dim3(kBlock),
                        sizeof(SharedWeights), st, gs.d_theta, gs.d_wbf, n16, ts);
     hipLaunchKernelGGL(k_train_wgrad, dim3(4 * splits), dim3(kBlock), 0, st, ts, n16, span);
@@ -490,7 +492,7 @@ spt_status guide_eval_launch(spt_ctx* ctx, const float* d_q, uint32_t stride, ui
     return check_hip(ctx, hipGetLastError(), "guide eval launch");
 }
 
-spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, float* loss_out) {
+spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, float* loss_out, uint32_t rs) {
     GuideState& gs = ctx->guide;
     spt_status ls = allow_lds(ctx);
     if (ls != SPT_OK) return ls;
@@ -500,7 +502,7 @@ spt_status guide_train_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, floa
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, st, ctl);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!gs.event_pair(e0, e1)) e0 = e1 = nullptr;
-    spt_status qs = queue_gradient(ctx, d_rec, n, e0, e1);
+    spt_status qs = queue_gradient(ctx, d_rec, n, rs, e0, e1);
     if (qs != SPT_OK) return qs;
     hipLaunchKernelGGL(k_guide_check, dim3(64), dim3(kBlock), 0, st, gs.d_grad, ctl);
     hipLaunchKernelGGL(k_guide_adam, dim3((kParams + kBlock - 1) / kBlock), dim3(kBlock), 0, st, gs.d_theta, gs.d_m,
@@ -523,7 +525,7 @@ spt_status guide_grad_launch(spt_ctx* ctx, const float* d_rec, uint32_t n, float
     hipStream_t st = ctx->stream;
     GuideCtl* ctl = gs.ctl();
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, st, ctl);
-    spt_status s = queue_gradient(ctx, d_rec, n, nullptr, nullptr);
+    spt_status s = queue_gradient(ctx, d_rec, n, 1u, nullptr, nullptr);
     if (s != SPT_OK) return s;
     s = check_hip(ctx, hipMemcpyAsync(grad_out, gs.d_grad, kParams * sizeof(float), hipMemcpyDeviceToHost, st),
                   "guide grad read");
@@ -663,7 +665,7 @@ spt_status spt_guide_train(spt_ctx* ctx, const float* d_records, uint32_t n, flo
     if (!ctx || !d_records || n == 0) return SPT_ERR_INVALID;
     if (!ctx->guide.ready) return set_error(ctx, SPT_ERR_STATE, "spt_guide_init first");
     if (!ctx->has_scene) return set_error(ctx, SPT_ERR_STATE, "the encoding needs the scene box: upload a scene");
-    return guide_train_launch(ctx, d_records, n, loss);
+    return guide_train_launch(ctx, d_records, n, loss, 1u);
 }
 
 spt_status spt_guide_gradient(spt_ctx* ctx, const float* d_records, uint32_t n, float* grad, float* loss) {
@@ -683,13 +685,14 @@ spt_status spt_guide_train_collected(spt_ctx* ctx, float* loss) {
     }
     const uint32_t n = (uint32_t)std::min<uint64_t>(gs.collected, gs.records_cap);
     spt_status s = SPT_OK;
-    // one step per batch; a remainder shorter than half a batch joins the last step
-    for (uint32_t b0 = 0; b0 < n && s == SPT_OK;) {
-        uint32_t len = std::min(gs.batch, n - b0);
-        if (n - b0 - len < gs.batch / 2) len = n - b0;
-        const bool last = b0 + len >= n;
-        s = guide_train_launch(ctx, gs.d_records + (size_t)b0 * kRecord, len, last ? loss : nullptr);
-        b0 += len;
+    // one step per batch (a remainder shorter than half a batch is spread over
+    // the others); step k takes records k, k + steps, k + 2 steps, ...: the
+    // records were appended in shading order, grouped by material and region,
+    // and interleaved batches sample the whole pass
+    const uint32_t steps = std::max<uint32_t>(1, n / gs.batch + ((n % gs.batch) >= gs.batch / 2 ? 1u : 0u));
+    for (uint32_t k = 0; k < steps && s == SPT_OK; ++k) {
+        const uint32_t len = (n - k + steps - 1) / steps;
+        s = guide_train_launch(ctx, gs.d_records + (size_t)k * kRecord, len, k + 1 == steps ? loss : nullptr, steps);
     }
     gs.collected = 0;
     return s;

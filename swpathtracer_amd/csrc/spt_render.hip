@@ -1103,7 +1103,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_de
                 if (now > a.rec_cap / 2) trained = std::max(trained, (now - a.rec_cap / 2 + bt - 1) / bt * bt);
                 while (s == SPT_OK && trained + bt <= upto) {
                     s = guide_train_launch(ctx, ctx->guide.d_records + (size_t)(trained % a.rec_cap) * guide::kRecord, bt,
-                                           nullptr);
+                                           nullptr, 1u);
                     trained += bt;
                 }
             }
