@@ -289,7 +289,7 @@ def main():
     ap.add_argument("--guide-lr", type=float, default=1e-2)
     ap.add_argument("--guide-warmup", type=int, default=4, help="training passes before the counted image")
     ap.add_argument("--guide-warmup-spp", type=int, default=4, help="spp of each warm-up training pass")
-    ap.add_argument("--guide-pass-spp", type=int, default=100, help="spp of each counted guided pass (a square)")
+    ap.add_argument("--guide-pass-spp", type=int, default=144, help="spp of each counted guided pass (a square)")
     ap.add_argument("--guide-steps", type=int, default=24, help="Adam steps per training pass")
     ap.add_argument("--guide-train-every", type=int, default=0,
                     help="> 0: train inside each guided call every this many bounces")
