@@ -163,6 +163,8 @@ def guide_experiment(args):
     # reference after each pass (comparison time not counted)
     dev.guide_init(seed=1, lr=args.guide_lr, batch=262144, steps_per_pass=args.guide_steps, flags=args.guide_flags,
                    train_every=args.guide_train_every)
+    if args.guide_capacity:  # queue of the guided passes (the plain render above used the default)
+        dev.set_tuning(path_capacity=args.guide_capacity)
     t_guided, t_train, hit = 0.0, 0.0, None
     curve = []
     k = 0
@@ -291,6 +293,7 @@ def main():
     ap.add_argument("--guide-warmup-spp", type=int, default=4, help="spp of each warm-up training pass")
     ap.add_argument("--guide-pass-spp", type=int, default=144, help="spp of each counted guided pass (a square)")
     ap.add_argument("--guide-steps", type=int, default=24, help="Adam steps per training pass")
+    ap.add_argument("--guide-capacity", type=int, default=0, help="path queue of the guided passes (0: default)")
     ap.add_argument("--guide-train-every", type=int, default=0,
                     help="> 0: train inside each guided call every this many bounces")
     ap.add_argument("--guide-flags", type=int, default=0,
