@@ -861,11 +861,11 @@ spt_status make_job(spt_ctx* ctx, const spt_camera* cam, const spt_render_params
 
 namespace {
 
-// Runs `total` paths through the wavefront: generation in chunks of the path
-// capacity, then bounces until no path is live (or cap_depth bounces).
+// Runs `total` paths through the wavefront: the queue is filled to its
+// capacity, then every bounce refills it from the paths not yet generated,
+// until every path is generated and none is live.
 template <bool MULTI>
-spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, int32_t cap_depth, const spt_tuning& tn,
-                     spt_render_stats& stats) {
+spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuning& tn, spt_render_stats& stats) {
     uint64_t cap = std::min<uint64_t>(total, tn.path_capacity);
     auto state_bytes = [](uint64_t c) {
         return (size_t)c * (2 * kBufBytes + kWorkBytes + 4 * 8) + (size_t)kMaxSortKeys * kMaxSortBlocks * 4 + 4096;
@@ -1206,7 +1206,7 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     a.keys = (!(prm->flags & SPT_RENDER_NO_SORT) && keys <= kMaxSortKeys) ? keys : 0u;
     spt_render_stats stats{};
     stats.paths = total;
-    s = run_paths<false>(ctx, a, total, a.job.max_depth, resolve_tuning(ctx->tuning), stats);
+    s = run_paths<false>(ctx, a, total, resolve_tuning(ctx->tuning), stats);
     stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->stats = stats;
     return s;
@@ -1217,7 +1217,6 @@ spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div) {
     const spt_tuning tn = resolve_tuning(ctx->tuning);
     const uint32_t nj = (uint32_t)b.jobs.size();
     std::vector<Job> jobs = b.jobs;
-    int32_t cap_depth = 0;
     for (Job& jb : jobs) {
         if (sample_div > 1) {  // tuning trial: a share of the samples, outputs untouched
             jb.spp_count = std::max<uint32_t>(1, jb.spp_count / sample_div);
@@ -1226,7 +1225,6 @@ spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div) {
             jb.path_rgb = nullptr;
         }
         jb.n_paths = (uint64_t)jb.vw * jb.vh * jb.spp_count;
-        if (cap_depth >= 0) cap_depth = (jb.max_depth < 0) ? -1 : std::max(cap_depth, jb.max_depth);
     }
     // generation tiles, taken round-robin from the jobs; within a job samples
     // advance slowest, so every pixel gets its first samples early
@@ -1310,7 +1308,7 @@ spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div) {
     a.keys = keys <= kMaxSortKeys ? keys : 0u;
     spt_render_stats stats{};
     stats.paths = total;
-    s = run_paths<true>(ctx, a, total, cap_depth, tn, stats);
+    s = run_paths<true>(ctx, a, total, tn, stats);
     if (s != SPT_OK) return s;
     std::vector<uint64_t> clk(nj + 1, 0);
     std::vector<uint32_t> done(nj, 0);
