@@ -339,7 +339,7 @@ typedef struct spt_tuning {
                                       a render allocates min(paths of the call, capacity), halved
                                       while the device has no room, down to 2^20) */
     uint32_t extend_blocks_per_cu; /* persistent traversal grid (default: occupancy limit) */
-    uint32_t shade_blocks_per_cu;  /* shading grid (default 16) */
+    uint32_t shade_blocks_per_cu;  /* shading grid, at most 1024 (default 128) */
     uint32_t use_graph;            /* 1: bounce loop captured once in a HIP graph and replayed;
                                       2: plain launches (default) */
     uint32_t tile_pixels;          /* multi-scene calls: pixels per generation tile (default 4096) */

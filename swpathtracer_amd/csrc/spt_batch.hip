@@ -77,7 +77,7 @@ spt_status tune(spt_ctx* ctx, bool batch, F run, spt_tuning* out) {
     std::vector<Dim> dims = {
         {&spt_tuning::path_capacity, {1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27}},
         {&spt_tuning::extend_blocks_per_cu, {0u, 6u, 4u, 3u, 2u}},
-        {&spt_tuning::shade_blocks_per_cu, {8u, 16u, 32u}},
+        {&spt_tuning::shade_blocks_per_cu, {16u, 32u, 64u, 128u}},
         {&spt_tuning::use_graph, {2u, 1u}},
         {&spt_tuning::check_every, {2u, 4u, 8u}},
         {&spt_tuning::refill_lanes, {16u, 32u, 48u, 56u}},
@@ -118,7 +118,7 @@ extern "C" {
 
 spt_status spt_set_tuning(spt_ctx* ctx, const spt_tuning* t) {
     if (!ctx || !t) return SPT_ERR_INVALID;
-    if (t->path_capacity > (1u << 27) || t->use_graph > 2 || t->shade_blocks_per_cu > 64 ||
+    if (t->path_capacity > (1u << 27) || t->use_graph > 2 || t->shade_blocks_per_cu > 1024 ||
         t->extend_blocks_per_cu > 64 || t->check_every > 64 || t->refill_lanes > 64)
         return set_error(ctx, SPT_ERR_INVALID, "tuning value out of range");
     ctx->tuning = *t;

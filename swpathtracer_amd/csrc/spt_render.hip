@@ -791,7 +791,7 @@ size_t guide_paths_lds_bytes() {
 spt_tuning resolve_tuning(const spt_tuning& t) {
     spt_tuning r = t;
     if (r.path_capacity == 0) r.path_capacity = 1u << 27;
-    if (r.shade_blocks_per_cu == 0) r.shade_blocks_per_cu = 16;
+    if (r.shade_blocks_per_cu == 0) r.shade_blocks_per_cu = 128;
     if (r.use_graph == 0) r.use_graph = 2;
     if (r.tile_pixels == 0) r.tile_pixels = 4096;
     if (r.tile_samples == 0) r.tile_samples = 16;
