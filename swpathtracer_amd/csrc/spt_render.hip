@@ -58,6 +58,7 @@ struct RenderArgs {
     uint32_t* job_done;         // paths finished per job
     uint64_t* job_clock;        // [0] start, [1 + j] finish of job j (device wall clock)
     uint32_t flags;
+    uint32_t per_path;  // some job writes per-path radiance: paths carry their output index
     uint32_t refill;  // idle lanes that trigger a ray refill in the traversal
     uint32_t keys;  // sort keys (n_mats + 1), 0 = no sort
     // learned guide
@@ -206,7 +207,7 @@ __device__ inline void gen_path(const RenderArgs& a, PathBuf& A, const Work& w, 
     A.thr[s] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(r.k));
     A.rad[s] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     A.key[s] = key;
-    A.local[s] = local;
+    if (a.per_path) A.local[s] = local;
     if (a.train) A.grec[s] = make_uint4(kNoRec, kNoRec, kNoRec, 0u);
 }
 
@@ -640,7 +641,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 rad.z += thr.z * jb.bg[2];
             }
             if (GUIDED && a.train) finish_records(a, A.grec[i], rad, A.key[i]);
-            finish_path<MULTI>(a, jb, jid, pix, A.local[i], rad);
+            finish_path<MULTI>(a, jb, jid, pix, a.per_path ? A.local[i] : 0ull, rad);
             continue;
         }
         const V3 o{o4.x, o4.y, o4.z}, d{d4.x, d4.y, d4.z};
@@ -737,7 +738,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         if (alive && (jb.max_depth >= 0) && ((int32_t)nd >= jb.max_depth)) alive = false;  // pathtracer.cpp:18
         if (!alive) {
             if (GUIDED && a.train) finish_records(a, A.grec[i], rad, A.key[i]);
-            finish_path<MULTI>(a, jb, jid, pix, A.local[i], rad);
+            finish_path<MULTI>(a, jb, jid, pix, a.per_path ? A.local[i] : 0ull, rad);
             continue;
         }
         // pathtracer.cpp:52-54, forwards: throughput *= f * cos / pdf
@@ -773,7 +774,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         B.thr[slot] = make_float4(thr.x * wgt.x, thr.y * wgt.y, thr.z * wgt.z, __uint_as_float(r.k));
         B.rad[slot] = make_float4(rad.x, rad.y, rad.z, __uint_as_float(nd));
         B.key[slot] = key;
-        B.local[slot] = A.local[i];
+        if (a.per_path) B.local[slot] = A.local[i];
         if (GUIDED && a.train) B.grec[slot] = gr;
     }
 }
@@ -1176,6 +1177,7 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     }
     a.job.root_leaf = ctx->root_leaf;
     a.flags = prm->flags;
+    a.per_path = a.job.path_rgb ? 1u : 0u;
     a.guided = (prm->flags & SPT_RENDER_GUIDED) ? 1u : 0u;
     a.train = (a.guided && (prm->flags & SPT_RENDER_GUIDE_TRAIN)) ? 1u : 0u;
     if ((prm->flags & SPT_RENDER_GUIDE_TRAIN) && !a.guided)
@@ -1304,6 +1306,7 @@ spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div) {
     a.job_done = b.d_done;
     a.job_clock = b.d_clock;
     a.flags = 0;
+    for (const Job& jb : jobs) a.per_path |= jb.path_rgb ? 1u : 0u;
     const uint32_t keys = b.n_mats + 1;
     a.keys = keys <= kMaxSortKeys ? keys : 0u;
     spt_render_stats stats{};
