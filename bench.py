@@ -162,7 +162,7 @@ def guide_experiment(args):
     # training; the running image of the counted passes is compared with the
     # reference after each pass (comparison time not counted)
     dev.guide_init(seed=1, lr=args.guide_lr, batch=262144, steps_per_pass=args.guide_steps, flags=args.guide_flags,
-                   train_every=args.guide_train_every)
+                   train_every=args.guide_train_every, bsdf_fraction=args.guide_bsdf_fraction)
     if args.guide_capacity:  # queue of the guided passes (the plain render above used the default)
         dev.set_tuning(path_capacity=args.guide_capacity)
     t_guided, t_train, hit = 0.0, 0.0, None
@@ -294,6 +294,7 @@ def main():
     ap.add_argument("--guide-pass-spp", type=int, default=144, help="spp of each counted guided pass (a square)")
     ap.add_argument("--guide-steps", type=int, default=24, help="Adam steps per training pass")
     ap.add_argument("--guide-capacity", type=int, default=0, help="path queue of the guided passes (0: default)")
+    ap.add_argument("--guide-bsdf-fraction", type=float, default=0.5, help="one-sample mixture weight of BSDF sampling")
     ap.add_argument("--guide-train-every", type=int, default=0,
                     help="> 0: train inside each guided call every this many bounces")
     ap.add_argument("--guide-flags", type=int, default=0,
