@@ -66,6 +66,11 @@ int orc_layer_eval(const spt_layer* l, const float* wo, const float* wi, float* 
 
 /* Exact conductor Fresnel reflectance for complex IOR eta + i k (GGX layer). */
 float orc_fresnel_conductor(float cos_i, float eta, float k);
+/* The float transcendentals both sides use: fn 0 sin, 1 cos, 2 exp, 3 log, 4 atan
+   of x[0..n) into y. */
+int orc_fmath(int fn, const float* x, float* y, uint64_t n);
+/* FresnelSchlick with two IORs (materials/fresnel_schlick.cpp:20-38), red channel. */
+float orc_schlick_ior(float ior_i, float ior_t, float cos_theta);
 
 /* Path tracing (integrators/pathtracer.cpp). rgb32: w*h*3 pixel values as the
  * reference leaves them (sum / spp, FP32); rgb64: same with FP64 sums; path_rgb:

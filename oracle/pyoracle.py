@@ -47,6 +47,8 @@ def load():
         "orc_material_sample": (i, [C.POINTER(L.Material), vp, u64, u64, vp]),
         "orc_layer_eval": (i, [C.POINTER(L.Layer), vp, vp, vp]),
         "orc_fresnel_conductor": (f, [f, f, f]),
+        "orc_fmath": (i, [i, vp, vp, u64]),
+        "orc_schlick_ior": (f, [f, f, f]),
         "orc_render": (i, [vp, C.POINTER(L.Camera), C.POINTER(L.RenderParams), vp, vp, vp, vp, i]),
         "orc_raycast": (i, [vp, C.POINTER(L.Camera), u32, vp, i]),
     }
@@ -194,6 +196,19 @@ def material_sample(mat, wo, seed, path):
 
 def fresnel_conductor(cos_i, eta, k):
     return load().orc_fresnel_conductor(cos_i, eta, k)
+
+
+def schlick_ior(ior_i, ior_t, cos_theta):
+    return load().orc_schlick_ior(ior_i, ior_t, cos_theta)
+
+
+def fmath(name, x):
+    """The float transcendentals the oracle and the device share."""
+    fn = ["sin", "cos", "exp", "log", "atan"].index(name)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    assert load().orc_fmath(fn, _p(x), _p(y), x.size) == 0
+    return y
 
 
 def layer_eval(layer, wo, wi):

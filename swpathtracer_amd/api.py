@@ -512,9 +512,10 @@ class Batch:
     """Many scenes rendered by one wavefront (spt_batch_*). `scenes` are
     SceneData-like objects (verts, normals, mat_ids, materials, camera,
     integrator). Each scene gets an FP32 film, and per-path radiance when
-    per_path is set."""
+    per_path is True or, given a collection of scene indices, for those
+    scenes only."""
 
-    def __init__(self, dev: Device, scenes, per_path: bool = False, film64: bool = False, n_threads: int = 8):
+    def __init__(self, dev: Device, scenes, per_path=False, film64: bool = False, n_threads: int = 8):
         self.dev, self.lib = dev, dev.lib
         self.scenes = list(scenes)
         self.bvhs, self.keep, self.films, self.films64, self.paths = [], [], [], [], []
@@ -539,7 +540,7 @@ class Batch:
                 p.flags |= L.SPT_RENDER_FILM64
             self.films64.append(f64)
             pp = None
-            if per_path:
+            if per_path is True or (not isinstance(per_path, bool) and i in per_path):
                 pp = dev.alloc(npix * sc.integrator.effective_spp() * 12)
                 p.flags |= L.SPT_RENDER_PER_PATH
             self.paths.append(pp)

@@ -112,7 +112,7 @@ __device__ inline V3 layer_f(const spt_layer& l, V3 wo, V3 wi) {
             const float nh2 = nh * nh, m2 = m * m;
             const float d1 = 1.0f / (kPi * m2 * nh2 * nh2);
             const float d2 = (nh2 - 1.0f) / (m2 * nh2);
-            const float d = (float)((double)d1 * d_exp((double)d2));  // ::exp(double) in the reference TU
+            const float d = d1 * f_exp(d2);
             const float g1 = 2.0f * nh / vh;
             const float g = gmin(1.0f, gmin(g1 * nv, g1 * nl));
             const V3 F = schlick_r0(l.r0, lh);
@@ -162,7 +162,9 @@ __device__ inline BSample sample_uniform(Draws& r) {
     const float r2 = r.next();
     const float phi = 2.0f * kPi * r2;
     const float s = sqrt_rn(1.0f - r1 * r1);
-    return BSample{V3{f_cos(phi) * s, r1, f_sin(phi) * s}, 1.0f / (2.0f * kPi)};
+    float sp, cp;
+    f_sincos(phi, sp, cp);
+    return BSample{V3{cp * s, r1, sp * s}, 1.0f / (2.0f * kPi)};
 }
 
 // CookTorranceBSDFSampler::getProbability (cook_torrance_bsdf_sampler.cpp:22-42)
@@ -198,7 +200,9 @@ __device__ inline BSample layer_sample(const spt_layer& l, V3 wo, Draws& r) {
             const float phi = 2.0f * kPi * r2;
             const float ct = sqrt_rn(r1);
             const float s = sqrt_rn(1.0f - ct * ct);
-            return BSample{V3{f_cos(phi) * s, ct, f_sin(phi) * s}, ct / kPi};
+            float sp, cp;
+            f_sincos(phi, sp, cp);
+            return BSample{V3{cp * s, ct, sp * s}, ct / kPi};
         }
         case SPT_BSDF_COOK_TORRANCE: {
             if (l.sampler == SPT_SAMPLER_UNIFORM) return sample_uniform(r);
@@ -207,8 +211,10 @@ __device__ inline BSample layer_sample(const spt_layer& l, V3 wo, Draws& r) {
             const float r2 = r.next();
             const float phi = 2.0f * kPi * r2;
             const float theta = f_atan(sqrt_rn(-(l.roughness * l.roughness) * f_log(1.0f - r1)));
-            const float st = f_sin(theta), ct = f_cos(theta);
-            const V3 m{f_cos(phi) * st, ct, f_sin(phi) * st};
+            float sp, cp, st, ct;
+            f_sincos(phi, sp, cp);
+            f_sincos(theta, st, ct);
+            const V3 m{cp * st, ct, sp * st};
             const V3 wi = 2.0f * m * dot(m, wo) - wo;
             return BSample{wi, beckmann_pdf(l, wo, wi)};
         }
@@ -247,7 +253,9 @@ __device__ inline BSample layer_sample(const spt_layer& l, V3 wo, Draws& r) {
             const float tan2 = a2 * r1 / (1.0f - r1);
             const float cm = 1.0f / sqrt_rn(1.0f + tan2);
             const float sm = sqrt_rn(gmax(0.0f, 1.0f - cm * cm));
-            const V3 m{f_cos(phi) * sm, cm, f_sin(phi) * sm};
+            float sp, cp;
+            f_sincos(phi, sp, cp);
+            const V3 m{cp * sm, cm, sp * sm};
             const V3 wi = 2.0f * m * dot(m, wo) - wo;
             float pdf = 1.0f;
             if (wi.y > 0.0f) {

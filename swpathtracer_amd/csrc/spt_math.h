@@ -12,6 +12,8 @@
 
 #include <cmath>
 
+#include "spt_fmath.h"
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define SPT_HD __host__ __device__ inline
@@ -153,30 +155,11 @@ SPT_HD float rng_u(uint64_t key, uint32_t k) {
     return (float)(z >> 40) * (1.0f / 16777216.0f);
 }
 
-// Float transcendentals. The reference calls glibc's float functions
-// (cosf, sinf, expf, logf, atanf), whose results are the correctly rounded
-// value in all but rare cases; on the device the double-precision routine is
-// evaluated and rounded once to float, which gives the same result in those
-// cases.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define SPT_F1(name, fn)                                       \
-    SPT_HD float name(float x) { return (float)::fn((double)x); }
-#else
-#define SPT_F1(name, fn) \
-    SPT_HD float name(float x) { return std::fn(x); }
-#endif
-SPT_F1(f_cos, cos)
-SPT_F1(f_sin, sin)
-SPT_F1(f_exp, exp)
-SPT_F1(f_log, log)
-SPT_F1(f_atan, atan)
-#undef SPT_F1
-SPT_HD double d_exp(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return ::exp(x);
-#else
-    return std::exp(x);
-#endif
-}
+// Float transcendentals: float-only routines that give the same bits on the
+// host and the device (spt_fmath.h); no FP64 on the device.
+SPT_HD void f_sincos(float x, float& s, float& c) { fm_sincos(x, s, c); }
+SPT_HD float f_exp(float x) { return fm_exp(x); }
+SPT_HD float f_log(float x) { return fm_log(x); }
+SPT_HD float f_atan(float x) { return fm_atan(x); }
 
 }  // namespace spt

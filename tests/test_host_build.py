@@ -205,3 +205,97 @@ def test_compressed_wide_tree_contains_the_exact_boxes(name):
     visit(0)
     assert np.all(seen_nodes == 1)
     assert len(got) == len(exp) and set(got) == exp
+
+
+def _sah_tree_check(v):
+    """Checks every node of the built tree against exhaustive enumeration of
+    the reference's object splits (acceleration/sbvh.cpp:58-121): for each
+    axis the primitives sorted by centroid, every prefix/suffix split, cost
+    2 * 1.2 + (A_L / A) N_L + (A_R / A) N_R with A the node's own (padded,
+    except at the root) box area (sbvh.cpp:47-56, 103-104); the node splits at
+    the cheapest one (first on ties, in the reference's scan order) iff it
+    costs less than N, and is a leaf otherwise. Computed here in float64 from
+    the triangles alone; nodes whose best and second-best costs are within
+    1e-5 relative (float rounding could order them either way) are skipped."""
+    b = api.Bvh(v, n_threads=1)
+    nodes, _, leaf, root = b.export()
+    tri = v.reshape(-1, 3, 3).astype(np.float32)
+    lo, hi = tri.min(1).astype(np.float64), tri.max(1).astype(np.float64)
+    cen = tri.astype(np.float64).sum(1) / 3.0  # vertex mean (triangle.cpp:85-87)
+
+    def area(mn, mx):
+        d = mx - mn
+        return 2.0 * (d[0] * (d[1] + d[2]) + d[1] * d[2])
+
+    checked = 0
+
+    def best_split(ids, box_area):
+        n = len(ids)
+        cands = []
+        for axis in range(3):
+            order = ids[np.argsort(cen[ids, axis], kind="stable")]
+            for i in range(n - 2, -1, -1):  # the reference's right-to-left sweep
+                L, R = order[:i + 1], order[i + 1:]
+                aL = area(lo[L].min(0), hi[L].max(0))
+                aR = area(lo[R].min(0), hi[R].max(0))
+                c = 2.4 + (aL / box_area) * len(L) + (aR / box_area) * len(R)
+                cands.append((c, axis, frozenset(L.tolist())))
+        return cands
+
+    def visit(ids, box, kind, ref):
+        nonlocal checked
+        n = len(ids)
+        if n == 1:  # no split exists
+            assert kind == "leaf"
+            return
+        cands = best_split(np.array(ids), area(np.asarray(box[0], np.float64), np.asarray(box[1], np.float64)))
+        costs = np.array([c for c, _, _ in cands])
+        best = int(np.argmin(costs))  # first minimum in scan order
+        srt = np.sort(costs)
+        ambiguous = len(srt) > 1 and abs(srt[1] - srt[0]) <= 1e-5 * srt[0] or abs(srt[0] - n) <= 1e-5 * n
+        if kind == "leaf":
+            if not ambiguous:
+                assert costs.min() >= n, (ids, costs.min())
+                checked += 1
+            return
+        nd = nodes[ref]
+        left = _ids_below(nd, "l")
+        if not ambiguous:
+            assert costs.min() < n
+            assert set(left) == set(cands[best][2]), (ids, left, cands[best])
+            checked += 1
+        for s in ("l", "r"):
+            sub = _ids_below(nd, s)
+            cbox = (nd[s + "min"], nd[s + "max"])
+            if nd[s + "cnt"] > 0:
+                visit(sub, cbox, "leaf", None)
+            else:
+                visit(sub, cbox, "inner", int(nd[s + "ref"]))
+
+    def _ids_below(nd, s):
+        if nd[s + "cnt"] > 0:
+            return leaf[nd[s + "ref"]:nd[s + "ref"] + nd[s + "cnt"]].tolist()
+        out = []
+        k = int(nd[s + "ref"])
+        for s2 in ("l", "r"):
+            out += _ids_below(nodes[k], s2)
+        return out
+
+    all_ids = list(range(len(tri)))
+    rbox = (root[:3], root[3:])
+    if len(nodes) == 0:
+        visit(all_ids, rbox, "leaf", None)
+    else:
+        visit(all_ids, rbox, "inner", 0)
+    return checked
+
+
+def test_sah_split_matches_exhaustive_enumeration():
+    rng = np.random.default_rng(17)
+    checked = 0
+    for trial in range(400):
+        n = int(rng.integers(2, 13))
+        c = rng.uniform(-1, 1, (n, 1, 3)) * rng.uniform(0.5, 4.0)
+        v = (c + rng.uniform(-0.4, 0.4, (n, 3, 3)) * rng.uniform(0.05, 1.0, (n, 1, 1))).reshape(n, 9)
+        checked += _sah_tree_check(v.astype(np.float32))
+    assert checked > 350  # nodes and leaves actually decided by the rule
