@@ -72,9 +72,16 @@ typedef struct spt_node64 {
     uint32_t lref, lcnt, rref, rcnt;
 } spt_node64;
 
-/* One triangle in leaf (depth-first, left-first) order: v0 and the edges
- * e1 = v1 - v0, e2 = v2 - v0 exactly as primitives/triangle.cpp:22-23 computes
- * them, plus the primitive id. 48 B = 3 x 16 B. */
+/* Primitive kinds (primitives/ directory of the reference). A primitive is
+ * 9 floats: a triangle's v0 v1 v2, or a sphere's centre xyz and radius
+ * (primitives/sphere.cpp) followed by 5 unused floats. */
+enum { SPT_PRIM_TRIANGLE = 0, SPT_PRIM_SPHERE = 1 };
+
+/* One primitive in leaf (depth-first, left-first) order, 48 B = 3 x 16 B.
+ * Triangle: v0 and the edges e1 = v1 - v0, e2 = v2 - v0 exactly as
+ * primitives/triangle.cpp:22-23 computes them. Sphere: v0 = centre,
+ * e1[0] = radius, other fields 0. prim_id: the primitive's scene index;
+ * pad[0]: material id (filled by spt_set_scene); pad[1]: SPT_PRIM_* kind. */
 typedef struct spt_tri48 {
     float v0[3];
     float e1[3];
@@ -209,6 +216,12 @@ typedef struct spt_ctx spt_ctx;
  * one the reference would build for the same primitive order. n_threads > 1
  * builds independent subtrees concurrently (same tree). */
 spt_status spt_bvh_build(const float* verts, uint32_t n_tris, uint32_t n_threads, spt_bvh** out);
+/* The same build over a scene of triangles and spheres in insertion order
+ * (core/scene.cpp:26-30): prims is n x 9 floats (see SPT_PRIM_*), kinds[i]
+ * the kind of primitive i (NULL: all triangles). A sphere's box is centre -+
+ * radius and its centroid the centre (primitives/sphere.cpp:47-60). */
+spt_status spt_bvh_build_prims(const float* prims, const uint8_t* kinds, uint32_t n, uint32_t n_threads,
+                               spt_bvh** out);
 spt_status spt_bvh_get_info(const spt_bvh* bvh, spt_bvh_info* info);
 /* Copies out the flattened tree: info.n_pair_nodes nodes, n_prims triangles,
  * and leaf_prim[i] = prim id of the i-th triangle in leaf order. Any pointer may
@@ -248,16 +261,20 @@ spt_status spt_memcpy_h2d(spt_ctx* ctx, void* d_dst, const void* src, size_t byt
 spt_status spt_memcpy_d2h(spt_ctx* ctx, void* dst, const void* d_src, size_t bytes);
 spt_status spt_memcpy_d2d(spt_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 
-/* Uploads a built tree (copied). normals: n_prims x 9 floats (n0 n1 n2, smooth
- * triangle, primitives/smooth_triangle.h:11-13) or NULL for flat triangles
- * (normal = normalize(cross(e1, e2)), primitives/triangle.cpp:5-6); a row of
- * nine zeros marks one flat triangle among smooth ones.
- * mat_ids: n_prims material indices or NULL (all 0). */
+/* Uploads a built tree (copied). verts: the primitives the tree was built
+ * over (n_prims x 9 floats, as spt_bvh_build / spt_bvh_build_prims took them).
+ * normals: n_prims x 9 floats (n0 n1 n2, smooth triangle,
+ * primitives/smooth_triangle.h:11-13) or NULL for flat triangles (normal =
+ * normalize(cross(e1, e2)), primitives/triangle.cpp:5-6); a row of nine zeros
+ * marks one flat triangle among smooth ones; a sphere's normal is always
+ * normalize(hit - centre) (primitives/sphere.cpp:35). mat_ids: n_prims
+ * material indices or NULL (all 0). */
 spt_status spt_set_scene(spt_ctx* ctx, const spt_bvh* bvh, const float* verts, const float* normals,
                          const uint32_t* mat_ids);
 /* Same, from the flattened arrays (as produced by spt_bvh_export). shading is
- * n_tris x 12 floats in leaf order: n0 n1 n2 (9), material id (as u32), smooth
- * flag (as u32), pad. */
+ * n_tris x 12 floats in leaf order: n0 n1 n2 (9), material id (as u32), normal
+ * kind (as u32: 0 flat n0, 1 smooth from n0 n1 n2, 2 sphere with centre n0),
+ * pad. */
 spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_nodes, const spt_tri48* tris,
                             uint32_t n_tris, const float* root_box, const float* shading);
 spt_status spt_set_materials(spt_ctx* ctx, const spt_material* mats, uint32_t n_mats);

@@ -68,10 +68,34 @@ class Scene:
     """Reference-semantics scene: primitives in insertion order + SBVH or brute force."""
 
     def __init__(self, verts=None, normals=None, mat_ids=None, spheres=None, sphere_mats=None, accel=1,
-                 materials=None):
+                 materials=None, kinds=None):
+        """Triangles (verts), then spheres (n, 4: centre, radius); or, with
+        kinds, verts holds n x 9 primitives of either kind in scene order
+        (a sphere row: centre, radius, unused)."""
         self.lib = load()
         self.h = self.lib.orc_scene_create(accel)
         self.n = 0
+        if kinds is not None:
+            v = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
+            k = np.asarray(kinds).reshape(-1)
+            nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+            mid = np.zeros(len(v), np.uint32) if mat_ids is None else np.ascontiguousarray(mat_ids, np.uint32)
+            i = 0
+            while i < len(v):  # runs of one kind, added in order
+                j = i
+                while j < len(v) and k[j] == k[i]:
+                    j += 1
+                m = np.ascontiguousarray(mid[i:j])
+                if k[i] == 1:
+                    sp = np.ascontiguousarray(v[i:j, 0:4])
+                    assert self.lib.orc_scene_add_spheres(self.h, _p(sp), _p(m), j - i) == 0
+                else:
+                    t = np.ascontiguousarray(v[i:j])
+                    n = None if nrm is None else np.ascontiguousarray(nrm[i:j])
+                    assert self.lib.orc_scene_add_triangles(self.h, _p(t), _p(n), _p(m), j - i) == 0
+                i = j
+            self.n = len(v)
+            verts = None
         if verts is not None:
             v = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
             n = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)

@@ -26,6 +26,8 @@ SPT_BSDF_COOK_TORRANCE = 2
 SPT_BSDF_SMOOTH_CONDUCTOR = 3
 SPT_BSDF_SMOOTH_DIELECTRIC = 4
 SPT_BSDF_GGX_CONDUCTOR = 5
+SPT_PRIM_TRIANGLE = 0
+SPT_PRIM_SPHERE = 1
 SPT_SAMPLER_IMPORTANCE = 0
 SPT_SAMPLER_UNIFORM = 1
 SPT_MAX_LAYERS = 4
@@ -133,6 +135,7 @@ assert C.sizeof(Layer) == 64 and C.sizeof(Material) == 272
 # name -> (restype, argtypes); every entry point declared in include/spt.h
 PROTOTYPES = {
     "spt_bvh_build": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "spt_bvh_build_prims": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "spt_bvh_get_info": (C.c_int32, [C.c_void_p, C.POINTER(BvhInfo)]),
     "spt_bvh_export": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "spt_bvh_dump": (C.c_int32, [C.c_void_p, C.c_char_p]),

@@ -145,14 +145,19 @@ assert NODE_DTYPE.itemsize == 64 and TRI_DTYPE.itemsize == 48
 
 
 class Bvh:
-    """Host SAH build over triangles (n, 9) float32 in scene order."""
+    """Host SAH build over primitives (n, 9) float32 in scene order: triangles
+    (v0 v1 v2), or, where kinds[i] == SPT_PRIM_SPHERE, spheres (centre xyz,
+    radius, 5 unused floats)."""
 
-    def __init__(self, verts: np.ndarray, n_threads: int = 8):
+    def __init__(self, verts: np.ndarray, n_threads: int = 8, kinds: Optional[np.ndarray] = None):
         self.lib = L.load()
         v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 9)
         self.verts = v
+        self.kinds = None if kinds is None else np.ascontiguousarray(kinds, np.uint8).reshape(-1)
+        if self.kinds is not None and self.kinds.shape[0] != v.shape[0]:
+            raise ValueError("one kind per primitive")
         h = C.c_void_p()
-        s = self.lib.spt_bvh_build(v.ctypes.data, v.shape[0], n_threads, C.byref(h))
+        s = self.lib.spt_bvh_build_prims(v.ctypes.data, _ptr(self.kinds), v.shape[0], n_threads, C.byref(h))
         if s != L.SPT_OK:
             raise SptError(s, "spt_bvh_build failed")
         self.h = h
@@ -521,7 +526,7 @@ class Batch:
         self.bvhs, self.keep, self.films, self.films64, self.paths = [], [], [], [], []
         arr = (L.BatchScene * len(self.scenes))()
         for i, sc in enumerate(self.scenes):
-            bvh = Bvh(sc.verts, n_threads=n_threads)
+            bvh = Bvh(sc.verts, n_threads=n_threads, kinds=getattr(sc, "kinds", None))
             self.bvhs.append(bvh)
             nrm = None if sc.normals is None else np.ascontiguousarray(sc.normals, np.float32).reshape(-1, 9)
             mid = np.ascontiguousarray(sc.mat_ids, np.uint32)

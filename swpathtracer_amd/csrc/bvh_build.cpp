@@ -167,6 +167,7 @@ struct spt_bvh {
     std::vector<spt_node64> nodes;
     std::vector<spt_tri48> tris;
     std::vector<uint32_t> leaf_prim;
+    std::vector<uint8_t> kinds;  // SPT_PRIM_* per scene primitive (empty: all triangles)
     float root_box[6] = {0, 0, 0, 0, 0, 0};
     std::unique_ptr<Node> root;  // kept for dump / statistics
 };
@@ -201,7 +202,7 @@ void collect_stats(spt_bvh* out) {  // sbvh.cpp:517-575
     }
 }
 
-void flatten(spt_bvh* out, const std::vector<Ref>& refs, const float* verts) {
+void flatten(spt_bvh* out, const std::vector<Ref>& refs, const float* verts, const uint8_t* kinds) {
     // Pre-order indices for inner nodes, leaf-order offsets for triangles.
     struct Item {
         Node* n;
@@ -270,11 +271,23 @@ void flatten(spt_bvh* out, const std::vector<Ref>& refs, const float* verts) {
             }
         }
     }
-    // triangles in leaf order, edges as triangle.cpp:22-23 computes them
+    // primitives in leaf order: triangle edges as triangle.cpp:22-23 computes
+    // them; a sphere's centre and radius
     const std::size_t nt = out->leaf_prim.size();
     out->tris.resize(nt);
     for (std::size_t i = 0; i < nt; ++i) {
         const float* p = verts + 9ull * out->leaf_prim[i];
+        if (kinds && kinds[out->leaf_prim[i]] == SPT_PRIM_SPHERE) {
+            spt_tri48& t = out->tris[i];
+            std::memset(&t, 0, sizeof(t));
+            t.v0[0] = p[0];
+            t.v0[1] = p[1];
+            t.v0[2] = p[2];
+            t.e1[0] = p[3];
+            t.prim_id = out->leaf_prim[i];
+            t.pad[1] = SPT_PRIM_SPHERE;
+            continue;
+        }
         V3 a{p[0], p[1], p[2]}, b{p[3], p[4], p[5]}, c{p[6], p[7], p[8]};
         V3 e1 = b - a, e2 = c - a;
         spt_tri48& t = out->tris[i];
@@ -289,7 +302,7 @@ void flatten(spt_bvh* out, const std::vector<Ref>& refs, const float* verts) {
         t.e2[2] = e2.z;
         t.prim_id = out->leaf_prim[i];
         t.pad[0] = 0;  // material id, filled when shading data is attached
-        t.pad[1] = 0;
+        t.pad[1] = SPT_PRIM_TRIANGLE;
     }
     out->info.n_pair_nodes = inner.size();
     out->info.max_depth = max_depth;
@@ -298,21 +311,41 @@ void flatten(spt_bvh* out, const std::vector<Ref>& refs, const float* verts) {
 }  // namespace
 
 extern "C" spt_status spt_bvh_build(const float* verts, uint32_t n_tris, uint32_t n_threads, spt_bvh** out) {
+    return spt_bvh_build_prims(verts, nullptr, n_tris, n_threads, out);
+}
+
+extern "C" spt_status spt_bvh_build_prims(const float* verts, const uint8_t* kinds, uint32_t n_tris,
+                                          uint32_t n_threads, spt_bvh** out) {
     if (!out || (!verts && n_tris) || n_tris == 0) return SPT_ERR_INVALID;
+    bool any_sphere = false;
+    if (kinds) {
+        for (uint32_t i = 0; i < n_tris; ++i) {
+            if (kinds[i] > SPT_PRIM_SPHERE) return SPT_ERR_INVALID;
+            any_sphere = any_sphere || kinds[i] == SPT_PRIM_SPHERE;
+        }
+    }
     auto t0 = std::chrono::steady_clock::now();
     std::unique_ptr<spt_bvh> bvh(new spt_bvh());
     try {
+        if (any_sphere) bvh->kinds.assign(kinds, kinds + n_tris);
         Builder bld;
         bld.threads_left = (int)(n_threads > 0 ? n_threads - 1 : 0);
         bld.refs.resize(n_tris);
         V3 rmn{0, 0, 0}, rmx{0, 0, 0};
         for (uint32_t i = 0; i < n_tris; ++i) {
             const float* p = verts + 9ull * i;
-            V3 a{p[0], p[1], p[2]}, b{p[3], p[4], p[5]}, c{p[6], p[7], p[8]};
             Ref& r = bld.refs[i];
-            r.mn = vmin(vmin(a, b), c);  // triangle.cpp:76-83
-            r.mx = vmax(vmax(a, b), c);
-            r.c = (a + b + c) * (1.0f / 3.0f);  // triangle.cpp:85-87
+            if (any_sphere && kinds[i] == SPT_PRIM_SPHERE) {
+                const V3 c{p[0], p[1], p[2]};
+                r.mn = c - p[3];  // sphere.cpp:47-55
+                r.mx = c + p[3];
+                r.c = c;  // sphere.cpp:58-60
+            } else {
+                V3 a{p[0], p[1], p[2]}, b{p[3], p[4], p[5]}, c{p[6], p[7], p[8]};
+                r.mn = vmin(vmin(a, b), c);  // triangle.cpp:76-83
+                r.mx = vmax(vmax(a, b), c);
+                r.c = (a + b + c) * (1.0f / 3.0f);  // triangle.cpp:85-87
+            }
             r.id = i;
             if (!i) {  // sbvh.cpp:20-23
                 rmn = r.mn;
@@ -336,7 +369,7 @@ extern "C" spt_status spt_bvh_build(const float* verts, uint32_t n_tris, uint32_
         bvh->root_box[4] = rmx.y;
         bvh->root_box[5] = rmx.z;
         collect_stats(bvh.get());
-        flatten(bvh.get(), bld.refs, verts);
+        flatten(bvh.get(), bld.refs, verts, any_sphere ? kinds : nullptr);
     } catch (const std::bad_alloc&) {
         return SPT_ERR_NOMEM;
     } catch (const std::exception&) {
@@ -405,4 +438,5 @@ const std::vector<spt_node64>& bvh_nodes(const spt_bvh* b) { return b->nodes; }
 const std::vector<spt_tri48>& bvh_tris(const spt_bvh* b) { return b->tris; }
 const std::vector<uint32_t>& bvh_leaf_prim(const spt_bvh* b) { return b->leaf_prim; }
 const float* bvh_root_box(const spt_bvh* b) { return b->root_box; }
+const std::vector<uint8_t>& bvh_kinds(const spt_bvh* b) { return b->kinds; }
 }  // namespace spt

@@ -103,10 +103,11 @@ void free_scene(spt_ctx* ctx) {
 
 template <typename T>
 spt_status upload(spt_ctx* ctx, T** dst, const void* src, size_t bytes, const char* what) {
-    if (bytes == 0) bytes = 16;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), bytes);
+    // an empty array still gets a (16-B) allocation so kernels have a valid
+    // address; only the caller's real bytes are copied
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), bytes ? bytes : 16);
     if (e != hipSuccess) return check_hip(ctx, e, what);
-    if (src) {
+    if (src && bytes) {
         e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
         if (e != hipSuccess) return check_hip(ctx, e, what);
     }
@@ -143,6 +144,13 @@ void scene_arrays(const spt_bvh* bvh, const float* verts, const float* normals, 
         const uint32_t m = mat_ids ? mat_ids[p] : 0u;
         tris[i].pad[0] = m;
         float* o = &sh[i * 12];
+        if (tris[i].pad[1] == SPT_PRIM_SPHERE) {  // normal from the hit point and the centre
+            const uint32_t kind = kShadeSphere;
+            std::memcpy(o, verts + 9ull * p, 3 * sizeof(float));
+            std::memcpy(&o[9], &m, 4);
+            std::memcpy(&o[10], &kind, 4);
+            continue;
+        }
         uint32_t smooth = 0u;
         if (normals)
             for (int j = 0; j < 9; ++j) smooth |= (normals[9ull * p + j] != 0.0f) ? 1u : 0u;
@@ -489,6 +497,8 @@ spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_no
                             uint32_t n_tris, const float* root_box, const float* shading) {
     if (!ctx || !tris || n_tris == 0 || !root_box || (n_nodes && !nodes)) return SPT_ERR_INVALID;
     if (n_nodes >= (1u << 30)) return set_error(ctx, SPT_ERR_UNSUPPORTED, "more than 2^30 nodes");
+    for (uint32_t i = 0; i < n_tris; ++i)
+        if (tris[i].pad[1] > SPT_PRIM_SPHERE) return set_error(ctx, SPT_ERR_INVALID, "unknown primitive kind");
     // every child reference must stay inside the arrays the kernels read
     for (uint32_t i = 0; i < n_nodes; ++i) {
         const spt_node64& nd = nodes[i];
@@ -523,11 +533,17 @@ spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_no
             flat.assign((size_t)n_tris * 12, 0.0f);
             for (uint32_t i = 0; i < n_tris; ++i) {
                 const spt_tri48& t = tris[i];
-                V3 n = normalize(cross(V3{t.e1[0], t.e1[1], t.e1[2]}, V3{t.e2[0], t.e2[1], t.e2[2]}));
                 float* o = &flat[(size_t)i * 12];
-                o[0] = n.x;
-                o[1] = n.y;
-                o[2] = n.z;
+                if (t.pad[1] == SPT_PRIM_SPHERE) {
+                    const uint32_t kind = kShadeSphere;
+                    std::memcpy(o, t.v0, 3 * sizeof(float));
+                    std::memcpy(&o[10], &kind, 4);
+                } else {
+                    V3 n = normalize(cross(V3{t.e1[0], t.e1[1], t.e1[2]}, V3{t.e2[0], t.e2[1], t.e2[2]}));
+                    o[0] = n.x;
+                    o[1] = n.y;
+                    o[2] = n.z;
+                }
                 uint32_t m = t.pad[0];
                 std::memcpy(&o[9], &m, 4);
             }
@@ -547,7 +563,11 @@ spt_status spt_upload_scene(spt_ctx* ctx, const spt_node64* nodes, uint32_t n_no
     ctx->n_leaves = (uint32_t)leaves.size();
     ctx->stack_cap4 = cap4;
     ctx->max_mat = 0;
-    for (uint32_t i = 0; i < n_tris; ++i) ctx->max_mat = std::max(ctx->max_mat, tris[i].pad[0]);
+    ctx->has_spheres = false;
+    for (uint32_t i = 0; i < n_tris; ++i) {
+        ctx->max_mat = std::max(ctx->max_mat, tris[i].pad[0]);
+        ctx->has_spheres = ctx->has_spheres || tris[i].pad[1] == SPT_PRIM_SPHERE;
+    }
     ctx->has_scene = true;
     return SPT_OK;
 }

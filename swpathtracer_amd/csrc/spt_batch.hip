@@ -214,6 +214,7 @@ spt_status spt_batch_create(spt_ctx* ctx, const spt_batch_scene* scenes, uint32_
             return fail(set_error(ctx, SPT_ERR_UNSUPPORTED, "batch exceeds 2^31 four-wide nodes or leaves"));
         for (spt_tri48 t : bt) {
             t.pad[0] += (uint32_t)mat_base;
+            d.has_spheres = d.has_spheres || t.pad[1] == SPT_PRIM_SPHERE;
             tris.push_back(t);
         }
         for (size_t k = 0; k < bt.size(); ++k) {

@@ -13,6 +13,8 @@
 namespace spt {
 
 constexpr int kBlock = 256;       // threads per block of the ray kernels (4 wave64)
+// normal kind of a shading record (word 10): flat n0, smooth n0 n1 n2, sphere centre n0
+constexpr uint32_t kShadeFlat = 0, kShadeSmooth = 1, kShadeSphere = 2;
 constexpr int kLdsStack = 16;     // traversal stack entries per lane kept in LDS
 constexpr uint32_t kNoHit = SPT_MISS;
 
@@ -81,7 +83,22 @@ struct DevScene {
     uint32_t n_mats;
     uint32_t stack_cap;  // traversal stack entries a lane may need (inner-node depth + 1)
     uint32_t stack_cap4; // same for the four-wide tree (3 per level + 1)
+    uint32_t has_spheres;  // some primitive record is a sphere (uniform branch in the leaf loops)
 };
+
+// Closest-hit test of one primitive record (a, b, c = its three float4):
+// Moller-Trumbore for a triangle, Hultquist for a sphere (centre a.xyz,
+// radius a.w, kind in c.w). u, v are the triangle's barycentrics (0 for a
+// sphere). `spheres` is the scene's uniform flag: without spheres the kind is
+// never read.
+__device__ inline bool prim_test(bool spheres, V3 o, V3 d, float4 a, float4 b, float4 c, float& t, float& u,
+                                 float& v) {
+    if (spheres && __float_as_uint(c.w) == SPT_PRIM_SPHERE) {
+        u = v = 0.0f;
+        return sphere_test(o, d, V3{a.x, a.y, a.z}, a.w, t);
+    }
+    return tri_test(o, d, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, t, u, v);
+}
 
 // Closest-hit result of one ray.
 struct Hit {
@@ -133,7 +150,7 @@ __device__ inline void leaf_test(const DevScene& sc, V3 o, V3 d, uint32_t first,
         const float4 a = tp[0], b = tp[1], c = tp[2];
         if (COUNT) ++tests;
         float t, u, v;
-        if (tri_test(o, d, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, t, u, v)) {
+        if (prim_test(sc.has_spheres != 0u, o, d, a, b, c, t, u, v)) {
             if (COUNT) ++isects;
             if ((t > 0.0f) && ((t < h.t) || ((t == h.t) && (k < h.idx)))) {
                 V3 p = o + d * t;
@@ -470,8 +487,8 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                         nc = tp[2];
                     }
                     float t, u, v;
-                    if (tri_test(L.o, L.d, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, t, u, v) &&
-                        (t > 0.0f) && ((t < L.h.t) || ((t == L.h.t) && (k < L.h.idx)))) {
+                    if (prim_test(sc.has_spheres != 0u, L.o, L.d, a, b, c, t, u, v) && (t > 0.0f) &&
+                        ((t < L.h.t) || ((t == L.h.t) && (k < L.h.idx)))) {
                         if (!have_box) {
                             const float4 b0 = box[0], b1 = box[1];
                             bmn = V3{b0.x, b0.y, b0.z};
@@ -517,14 +534,17 @@ __device__ inline void camera_ray(const DevCamera& c, float x, float y, V3& o, V
     }
 }
 
-// Shading normal of a hit: SmoothTriangle::getNormal
-// (primitives/smooth_triangle.h:11-13) or the stored flat normal
-// (primitives/triangle.cpp:5-6).
-__device__ inline V3 shading_normal(const DevScene& sc, const Hit& h) {
+// Shading normal of a hit at pos: SmoothTriangle::getNormal
+// (primitives/smooth_triangle.h:11-13), the stored flat normal
+// (primitives/triangle.cpp:5-6) or a sphere's normalize(pos - centre)
+// (primitives/sphere.cpp:35).
+__device__ inline V3 shading_normal(const DevScene& sc, const Hit& h, V3 pos) {
     const float4* sp = sc.shading + 3 * (size_t)h.idx;
     const float4 a = sp[0], b = sp[1], c = sp[2];
     const V3 n0{a.x, a.y, a.z};
-    if (__float_as_uint(c.z) == 0u) return n0;
+    const uint32_t kind = __float_as_uint(c.z);
+    if (kind == kShadeFlat) return n0;
+    if (kind == kShadeSphere) return normalize(pos - n0);
     const V3 n1{a.w, b.x, b.y}, n2{b.z, b.w, c.x};
     return normalize(n0 * (1.0f - h.u - h.v) + n1 * h.u + n2 * h.v);
 }
@@ -586,6 +606,7 @@ struct BatchData {
     float4* d_shading = nullptr;
     spt_material* d_mats = nullptr;
     uint32_t n_nodes = 0, n_tris = 0, n_mats = 0, stack_cap = 0, stack_cap4 = 0;
+    bool has_spheres = false;
     std::vector<Job> jobs;  // host copy: sampling, outputs, roots
     Job* d_jobs = nullptr;
     BatchTile* d_tiles = nullptr;
@@ -704,6 +725,7 @@ struct spt_ctx {
     uint32_t root_leaf = 0;
     uint32_t stack_cap = 0;
     uint32_t max_mat = 0;  // largest material id referenced by a triangle
+    bool has_spheres = false;
     bool has_scene = false;
     std::vector<spt_material> h_mats;
     uint8_t* d_guidable = nullptr;  // per material: kGuideSingle / kGuideCoated, 0 = not guide-sampled
@@ -747,6 +769,7 @@ struct spt_ctx {
         s.n_nodes = n_nodes;
         s.n_mats = n_mats;
         s.stack_cap = stack_cap;
+        s.has_spheres = has_spheres ? 1u : 0u;
         return s;
     }
 };

@@ -16,6 +16,8 @@ const std::vector<spt_node64>& bvh_nodes(const spt_bvh* b);
 const std::vector<spt_tri48>& bvh_tris(const spt_bvh* b);
 const std::vector<uint32_t>& bvh_leaf_prim(const spt_bvh* b);
 const float* bvh_root_box(const spt_bvh* b);
+// SPT_PRIM_* of every scene primitive; empty when all are triangles.
+const std::vector<uint8_t>& bvh_kinds(const spt_bvh* b);
 
 // Camera state precomputed on the host exactly as the reference's camera
 // constructors do (cameras/camera.cpp:3-11, pinhole_camera.cpp:3-12,

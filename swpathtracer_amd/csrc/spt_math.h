@@ -142,6 +142,20 @@ SPT_HD bool tri_test(V3 o, V3 d, V3 v0, V3 e1, V3 e2, float& t, float& u, float&
     return det_ok & u_ok & v_ok;
 }
 
+// Sphere::intersect (primitives/sphere.cpp:6-38, Hultquist 1990): a hit
+// whenever the line meets the sphere; t is the near root when it exceeds
+// 1e-5, else the far one (possibly negative: the caller's t > 0 test
+// rejects it).
+SPT_HD bool sphere_test(V3 o, V3 d, V3 c, float r, float& t) {
+    const V3 eo = c - o;
+    const float v = dot(eo, d);
+    const float disc = (r * r) - (dot(eo, eo) - (v * v));
+    const float sq = sqrt_rn(disc);  // NaN when disc < 0, rejected below
+    const float t1 = v - sq, t2 = v + sq;
+    t = (t1 > 1e-5f) ? t1 : t2;
+    return !(disc < 0.0f);
+}
+
 // Counter-based draws (definition in include/spt.h).
 static constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
 SPT_HD uint64_t mix64(uint64_t z) {

@@ -453,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_guide_paths(RenderArgs a, PathBuf A,
                 h.u = hv.z;
                 h.v = hv.w;
                 const V3 pos = V3{o4.x, o4.y, o4.z} + d * h.t;
-                encode(pos, shading_normal(a.sc, h), -d, box.bmin, box.bext, f);
+                encode(pos, shading_normal(a.sc, h, pos), -d, box.bmin, box.bext, f);
             }
         }
         if (!mine)
@@ -651,7 +651,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         h.u = hv.z;
         h.v = hv.w;
         const V3 pos = o + d * h.t;  // triangle.cpp:69
-        const V3 nrm = shading_normal(a.sc, h);
+        const V3 nrm = shading_normal(a.sc, h, pos);
         const uint32_t mid = w.skey[i];
         const spt_material& m = lds_mats ? smats[mid] : a.sc.mats[mid];
         const Frame fr = frame_from_up(nrm);          // pathtracer.cpp:28
@@ -1299,6 +1299,7 @@ spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div) {
     a.sc.n_nodes = b.n_nodes;
     a.sc.n_mats = b.n_mats;
     a.sc.stack_cap = b.stack_cap;
+    a.sc.has_spheres = b.has_spheres ? 1u : 0u;
     a.jobs = b.d_jobs;
     a.tiles = b.d_tiles;
     a.tile_start = b.d_tile_start;

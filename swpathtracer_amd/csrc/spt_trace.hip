@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_raycast(DevScene sc, DevCamera cam, 
             if (KIND == SPT_RAYCAST_DEPTH) {
                 reinterpret_cast<float*>(out)[i] = (h.idx != kNoHit) ? h.t : -1.0f;
             } else {
-                V3 nrm = (h.idx != kNoHit) ? shading_normal(sc, h) : V3{0.0f, 0.0f, 0.0f};
+                V3 nrm = (h.idx != kNoHit) ? shading_normal(sc, h, o + d * h.t) : V3{0.0f, 0.0f, 0.0f};
                 float* o3 = reinterpret_cast<float*>(out) + 3 * i;
                 o3[0] = nrm.x;
                 o3[1] = nrm.y;

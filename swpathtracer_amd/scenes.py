@@ -26,6 +26,7 @@ class SceneData:
     camera: Optional[L.Camera] = None
     integrator: Optional[api.Integrator] = None
     name: str = ""
+    kinds: Optional[np.ndarray] = None  # (n,) uint8 SPT_PRIM_* per primitive; None: all triangles
 
     @property
     def n_tris(self) -> int:
@@ -39,7 +40,20 @@ def concat(parts: List[SceneData], materials: List[L.Material], name="") -> Scen
     else:
         normals = None
     mats = np.concatenate([p.mat_ids for p in parts]).astype(np.uint32)
-    return SceneData(verts, normals, mats, materials, name=name)
+    kinds = None
+    if any(p.kinds is not None for p in parts):
+        kinds = np.concatenate([p.kinds if p.kinds is not None else np.zeros(p.n_tris, np.uint8) for p in parts])
+    return SceneData(verts, normals, mats, materials, name=name, kinds=kinds)
+
+
+def spheres(centres_radii, mats) -> SceneData:
+    """Sphere primitives (primitives/sphere.cpp): rows (cx, cy, cz, r) as 9-float
+    primitive rows with kind SPT_PRIM_SPHERE."""
+    cr = np.asarray(centres_radii, np.float32).reshape(-1, 4)
+    v = np.zeros((cr.shape[0], 9), np.float32)
+    v[:, 0:4] = cr
+    return SceneData(v, None, np.asarray(mats, np.uint32).reshape(-1), [],
+                     kinds=np.full(cr.shape[0], L.SPT_PRIM_SPHERE, np.uint8))
 
 
 # ----------------------------------------------------------------------------
@@ -319,6 +333,36 @@ def furnace(subdiv: int = 3, albedo=(0.999, 0.999, 0.999), layer=None, width=64,
     sc.camera = api.orthographic(width, height, -1.25, 1.25, -1.25, 1.25, position=(0, 0, 6))
     sc.integrator = api.Integrator(spp=spp, pixel_sampler="jittered", termination="russian-roulette",
                                    path_length=3, seed=7)
+    return sc
+
+
+def reference_furnace_materials():
+    """The materials samples/furnace_test.lua:48-133 defines (importance sampling)."""
+    perf = (0.999, 0.999, 0.999)
+    return {
+        "lambertian": api.material([api.lambertian(perf)]),
+        "smooth_conductor": api.material([api.smooth_conductor(perf)]),
+        "smooth_dielectric": api.material([api.smooth_dielectric(1.49)]),
+        "cook_torrance": api.material([api.cook_torrance(0.2, perf)]),
+        "plastic": api.material([api.smooth_dielectric(1.49), api.lambertian(perf)]),
+        "coating": api.material([api.smooth_dielectric(1.49), api.cook_torrance(0.2, perf)]),
+    }
+
+
+def reference_furnace(material: str = "cook_torrance", width: int = 512, height: int = 512, spp: int = 256,
+                      seed: int = 0) -> SceneData:
+    """samples/furnace_test.lua as written: orthographic camera at (0, 0, 6)
+    over [-1.25, 1.25]^2, jittered spp, Russian roulette from path length 3,
+    background 0; an emitter sphere of radius 10 (kd 0, emission 0.5) around a
+    unit sphere of the chosen material (the script's default: Cook-Torrance)."""
+    light = api.material([api.lambertian((0.0, 0.0, 0.0))], emission=(0.5, 0.5, 0.5))
+    mats = [light, reference_furnace_materials()[material]]
+    sc = spheres([[0, 0, 0, 10], [0, 0, 0, 1]], [0, 1])
+    sc.materials = mats
+    sc.name = f"reference_furnace_{material}"
+    sc.camera = api.orthographic(width, height, -1.25, 1.25, -1.25, 1.25, position=(0, 0, 6))
+    sc.integrator = api.Integrator(spp=spp, pixel_sampler="jittered", termination="russian-roulette",
+                                   path_length=3, seed=seed)
     return sc
 
 

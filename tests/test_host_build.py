@@ -299,3 +299,34 @@ def test_sah_split_matches_exhaustive_enumeration():
         v = (c + rng.uniform(-0.4, 0.4, (n, 3, 3)) * rng.uniform(0.05, 1.0, (n, 1, 1))).reshape(n, 9)
         checked += _sah_tree_check(v.astype(np.float32))
     assert checked > 350  # nodes and leaves actually decided by the rule
+
+
+def _mixed_scene(seed=5, n_spheres=40):
+    """Box triangles, random spheres and an icosphere, interleaved in scene order."""
+    rng = np.random.default_rng(seed)
+    box = scenes.cornell_icosphere(subdiv=2)
+    sp = np.concatenate([rng.uniform(-0.8, 0.8, (n_spheres, 3)), rng.uniform(0.02, 0.3, (n_spheres, 1))], 1)
+    parts = [scenes.concat([box], []), scenes.spheres(sp[: n_spheres // 2], np.arange(n_spheres // 2) % 5),
+             scenes.icosphere(1, (0.3, 0.2, -0.1), 0.2, mat=2), scenes.spheres(sp[n_spheres // 2:], 3)]
+    return scenes.concat(parts, box.materials)
+
+
+def test_tree_with_spheres_equals_reference_tree(tmp_path):
+    """Spheres enter the build with box centre -+ radius and centroid = centre
+    (primitives/sphere.cpp:47-60); with triangles interleaved in scene order
+    the tree, leaf order and primitive records match the oracle's SBVH."""
+    sc = _mixed_scene()
+    assert sc.kinds is not None and sc.kinds.sum() == 40
+    b = api.Bvh(sc.verts, n_threads=4, kinds=sc.kinds)
+    o = O.Scene(sc.verts, sc.normals, sc.mat_ids, kinds=sc.kinds)
+    b.dump(str(tmp_path / "p.dat"))
+    o.dump(str(tmp_path / "o.dat"))
+    assert (tmp_path / "p.dat").read_text() == (tmp_path / "o.dat").read_text()
+    _, tris, leaf, _ = b.export()
+    assert np.array_equal(leaf, o.leaf_order())
+    is_sp = sc.kinds[leaf] == L.SPT_PRIM_SPHERE
+    assert np.array_equal(tris["pad"][:, 1], sc.kinds[leaf].astype(np.uint32))
+    assert np.array_equal(tris["v0"][is_sp], sc.verts[leaf[is_sp], 0:3])
+    assert np.array_equal(tris["e1"][is_sp][:, 0], sc.verts[leaf[is_sp], 3])
+    with pytest.raises(api.SptError):
+        api.Bvh(sc.verts, kinds=np.full(sc.n_tris, 7, np.uint8))
