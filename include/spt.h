@@ -511,6 +511,72 @@ spt_status spt_guide_get_stats(spt_ctx* ctx, spt_guide_stats* out);
 /* Device pointer and count of the collected records. */
 spt_status spt_guide_records(spt_ctx* ctx, float** d_records, uint32_t* n);
 
+/* ------------------------------------------------------------------------- */
+/* Host: Lua scene scripts (replaces lb::LuaBind, lua_bind/lua_bind.cpp, and  */
+/* the scene assembly of SwpathtracerApp, core/swpathtracer_app.cpp:40-262)   */
+/* ------------------------------------------------------------------------- */
+
+/* What the script's Integrator{} selects (lua_core/integrators.lua). */
+enum {
+    SPT_LUA_NONE = 0,
+    SPT_LUA_PATHTRACER = 1,     /* type = "path-tracer"                            */
+    SPT_LUA_RAYCAST_NORMAL = 2, /* type = "ray-caster", pixel_value = "normal"     */
+    SPT_LUA_RAYCAST_DEPTH = 3,  /* ... "depth" (min_value/max_value: minimum/maximum_depth) */
+    SPT_LUA_RAYCAST_COUNT = 4   /* ... "intersection-test-count" (minimum/maximum_count)   */
+};
+/* GlobalSettings{acceleration_structure} (lua_bind/lua_bind.cpp:122-155). */
+enum { SPT_ACCEL_NONE = 0, SPT_ACCEL_BVH_SAH = 1, SPT_ACCEL_SBVH_SAH = 2 };
+/* spt_lua_load flags: the scene functions of the lua_core scripts come from the
+ * library's built-in restatement even when package.path would find the files. */
+#define SPT_LUA_BUILTIN_CORE 1u
+
+typedef struct spt_lua_info {
+    spt_camera camera;         /* Camera{} with ImageBuffer{}'s width and height */
+    spt_render_params params;  /* PixelSampler{}, path termination, viewport, background;
+                                  FILM32, counter RNG, seed 0 */
+    uint32_t integrator;       /* SPT_LUA_* */
+    uint32_t accel;            /* SPT_ACCEL_* */
+    float overlap_threshold;   /* inf for bvh-sah (lua_core/global_settings.lua:14-16) */
+    float min_value, max_value;/* depth / count ray caster ranges (-1: from the image) */
+    int32_t path_length;       /* Integrator{path_length} */
+    uint64_t n_prims;          /* primitives after mesh expansion, scene order */
+    uint64_t n_materials;
+    uint64_t n_objects;        /* Triangle/SmoothTriangle/Sphere/Mesh statements */
+    uint64_t n_bsdfs;
+    char file_name[256];       /* ImageBuffer{file_name} */
+} spt_lua_info;
+
+typedef struct spt_lua_scene spt_lua_scene;
+
+/* Runs a scene script as lb::LuaBind::runScript does (lua_bind.cpp:18-31) on
+ * the library's own interpreter of the Lua 5.3 subset scene scripts use, then
+ * extracts the scene as lua_bind's getters and SwpathtracerApp's loaders do.
+ * Relative paths (the script, package.path entries, Mesh{filename}) resolve
+ * against workdir (NULL: the process's working directory), the directory the
+ * reference's app runs from. `require "core"` loads the lua_core scripts through
+ * package.path; when no file is found (or with SPT_LUA_BUILTIN_CORE) the
+ * library's restatement of those functions is used: same defaults,
+ * validation messages and registry layout. math.random() is Lua 5.3's on
+ * glibc (random() / 2^31, unseeded = srandom(1)). A validation failure (the
+ * reference prints "Lua ERROR: ..." and exits) or a lua_bind error (the
+ * reference throws) returns SPT_ERR_INVALID with the message in err. */
+spt_status spt_lua_load(const char* script, const char* workdir, uint32_t flags, spt_lua_scene** out, char* err,
+                        size_t err_len);
+spt_status spt_lua_get_info(const spt_lua_scene* sc, spt_lua_info* info);
+/* prims/normals: n_prims x 9 floats; kinds: n_prims SPT_PRIM_* (spt_bvh_build_prims,
+ * spt_set_scene); mat_ids: n_prims. Any pointer may be NULL. */
+spt_status spt_lua_get_prims(const spt_lua_scene* sc, float* prims, uint8_t* kinds, float* normals,
+                             uint32_t* mat_ids);
+/* n_materials records for spt_set_materials: Material{bsdf = LayeredBSDF{...}}
+ * layers top first, with each BSDF's sampler as the app picks it. */
+spt_status spt_lua_get_materials(const spt_lua_scene* sc, spt_material* mats);
+/* The registry (_G.__RENDERER_ELEMENTS__) as JSON text (keys sorted, array
+ * entries as "#000000000001"), and what the script printed. *len: capacity
+ * on entry (buf may be NULL to query), length + 1 on return. */
+spt_status spt_lua_registry_json(const spt_lua_scene* sc, char* buf, uint64_t* len);
+spt_status spt_lua_printed(const spt_lua_scene* sc, char* buf, uint64_t* len);
+void spt_lua_destroy(spt_lua_scene* sc);
+
 #ifdef __cplusplus
 }
 #endif

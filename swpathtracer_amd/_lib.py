@@ -57,6 +57,15 @@ SPT_TRACE_FETCH = 3
 SPT_RAYCAST_NORMAL = 0
 SPT_RAYCAST_DEPTH = 1
 SPT_RAYCAST_COUNT = 2
+SPT_LUA_NONE = 0
+SPT_LUA_PATHTRACER = 1
+SPT_LUA_RAYCAST_NORMAL = 2
+SPT_LUA_RAYCAST_DEPTH = 3
+SPT_LUA_RAYCAST_COUNT = 4
+SPT_ACCEL_NONE = 0
+SPT_ACCEL_BVH_SAH = 1
+SPT_ACCEL_SBVH_SAH = 2
+SPT_LUA_BUILTIN_CORE = 1
 
 F3 = C.c_float * 3
 
@@ -130,6 +139,13 @@ class BatchStats(C.Structure):
                 ("device_bytes", C.c_uint64)]
 
 
+class LuaInfo(C.Structure):
+    _fields_ = [("camera", Camera), ("params", RenderParams), ("integrator", C.c_uint32), ("accel", C.c_uint32),
+                ("overlap_threshold", C.c_float), ("min_value", C.c_float), ("max_value", C.c_float),
+                ("path_length", C.c_int32), ("n_prims", C.c_uint64), ("n_materials", C.c_uint64),
+                ("n_objects", C.c_uint64), ("n_bsdfs", C.c_uint64), ("file_name", C.c_char * 256)]
+
+
 assert C.sizeof(Layer) == 64 and C.sizeof(Material) == 272
 
 # name -> (restype, argtypes); every entry point declared in include/spt.h
@@ -182,6 +198,14 @@ PROTOTYPES = {
     "spt_batch_get_stats": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(BatchStats), C.c_void_p]),
     "spt_batch_autotune": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(Tuning)]),
     "spt_batch_destroy": (None, [C.c_void_p]),
+    "spt_lua_load": (C.c_int32, [C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(C.c_void_p), C.c_char_p,
+                                 C.c_size_t]),
+    "spt_lua_get_info": (C.c_int32, [C.c_void_p, C.POINTER(LuaInfo)]),
+    "spt_lua_get_prims": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "spt_lua_get_materials": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "spt_lua_registry_json": (C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint64)]),
+    "spt_lua_printed": (C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint64)]),
+    "spt_lua_destroy": (None, [C.c_void_p]),
 }
 
 _lib = None

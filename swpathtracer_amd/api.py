@@ -305,8 +305,13 @@ class Device:
 
     # scene -------------------------------------------------------------
     def set_scene(self, bvh: Bvh, normals: Optional[np.ndarray] = None, mat_ids: Optional[np.ndarray] = None):
+        n_prims = bvh.verts.shape[0]
         n = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
-        m = None if mat_ids is None else np.ascontiguousarray(mat_ids, np.uint32)
+        m = None if mat_ids is None else np.ascontiguousarray(mat_ids, np.uint32).reshape(-1)
+        if n is not None and n.shape[0] != n_prims:
+            raise ValueError(f"normals has {n.shape[0]} rows for {n_prims} primitives")
+        if m is not None and m.shape[0] != n_prims:
+            raise ValueError(f"mat_ids has {m.shape[0]} entries for {n_prims} primitives")
         self._check(self.lib.spt_set_scene(self.h, bvh.h, bvh.verts.ctypes.data, _ptr(n), _ptr(m)))
 
     def set_materials(self, mats: Sequence[L.Material]):
@@ -529,7 +534,9 @@ class Batch:
             bvh = Bvh(sc.verts, n_threads=n_threads, kinds=getattr(sc, "kinds", None))
             self.bvhs.append(bvh)
             nrm = None if sc.normals is None else np.ascontiguousarray(sc.normals, np.float32).reshape(-1, 9)
-            mid = np.ascontiguousarray(sc.mat_ids, np.uint32)
+            mid = np.ascontiguousarray(sc.mat_ids, np.uint32).reshape(-1)
+            if mid.shape[0] != bvh.verts.shape[0] or (nrm is not None and nrm.shape[0] != bvh.verts.shape[0]):
+                raise ValueError(f"scene {i}: mat_ids/normals do not have one row per primitive")
             mats = (L.Material * len(sc.materials))(*sc.materials)
             self.keep += [nrm, mid, mats]
             cam, p = sc.camera, sc.integrator.params()

@@ -52,7 +52,8 @@ def spheres(centres_radii, mats) -> SceneData:
     cr = np.asarray(centres_radii, np.float32).reshape(-1, 4)
     v = np.zeros((cr.shape[0], 9), np.float32)
     v[:, 0:4] = cr
-    return SceneData(v, None, np.asarray(mats, np.uint32).reshape(-1), [],
+    m = np.broadcast_to(np.asarray(mats, np.uint32).reshape(-1), (cr.shape[0],)).copy()
+    return SceneData(v, None, m, [],
                      kinds=np.full(cr.shape[0], L.SPT_PRIM_SPHERE, np.uint8))
 
 

@@ -87,20 +87,23 @@ def test_pixel_sampler_replay_parity(device, sampler, spp):
 
 def test_fp64_film_matches_oracle_fp64_image(device):
     """SPT_RENDER_FILM64: per-pixel FP64 sums of the FP32 path radiances. The
-    oracle sums the same per-path values in double in sample order; the device
-    adds them with FP64 atomics in any order, so the two sums differ only by
-    double rounding (far below 1e-12 relative)."""
+    device adds its own per-path values with FP64 atomics in any order, so the
+    film equals the float64 sum of its per-path output up to double rounding
+    (1e-12 relative). Against the oracle's FP64 image the bound is the
+    per-path one (1e-5 relative): the paths agree to 1e-5, not bit for bit."""
     sc = _box(spp=25)
     orc = _setup(device, sc)
     img64, paths = device.render_image(sc.camera, sc.integrator, film64=True, per_path=True)
     _, img64_c, paths_c, _ = orc.render(sc.camera, sc.integrator.params(), rgb64=True, per_path=True)
     assert img64.dtype == np.float64
     assert _rel_ok(paths, paths_c).all()
-    np.testing.assert_allclose(img64, img64_c, rtol=1e-12, atol=1e-300)
     # the same paths summed from the device's own per-path output in float64
     spp = sc.integrator.effective_spp()
     sums = paths.astype(np.float64).reshape(32, 40, spp, 3).sum(2) / spp
     np.testing.assert_allclose(img64, sums, rtol=1e-12, atol=1e-300)
+    # the oracle's FP64 film of its own paths: within the per-path tolerance
+    np.testing.assert_allclose(img64, img64_c, rtol=RTOL, atol=1e-30)
+    assert np.abs(img64 - img64_c).max() > 0 or np.array_equal(paths, paths_c)
     img32, _ = device.render_image(sc.camera, sc.integrator)
     np.testing.assert_allclose(img32, img64, rtol=2e-5, atol=1e-7)
 
