@@ -1,16 +1,28 @@
 """Benchmark: plain GPU path tracing on one MI355X.
 
-Workload: the 1,048,576-triangle Perlin-displaced sphere with a 2-layer BSDF
-(smooth dielectric, IOR 1.5, over GGX alpha 0.2 gold) inside the Lambertian
-box with an emissive ceiling quad, 800x600, jittered, Russian roulette from
-depth 3, max depth 16. One step = 16 samples of every pixel (7.68M camera
-paths); the default 64 steps render the full 1024 spp. Inputs (tree,
-triangles, materials) are resident in HBM before timing.
+Workload (the value): the 1,048,576-triangle Perlin-displaced sphere with a
+2-layer BSDF (smooth dielectric, IOR 1.5, over GGX alpha 0.2 gold) inside the
+Lambertian box with an emissive ceiling quad, 800x600, jittered, Russian
+roulette from depth 3, max depth 16. One step = one render call of the full
+1024-spp image (491.52M camera paths); the default is 1 timed step after 1
+warm-up step. Inputs (tree, triangles, materials) are resident in HBM before
+timing; the value is camera paths per second over the timed steps.
 
-Also reported (same JSON line): closest-hit throughput on 16,777,216
-incoherent rays over the same mesh, the roofline of the dominant kernel (the
-closest-hit extension kernel inside the path tracer) and the oracle (the CPU
-restatement of the reference's OpenMP path tracer) timed on the host cores.
+Also reported in the same JSON line:
+- roofline of the dominant kernel (k_extend, the closest-hit traversal of
+  extension rays) from its HIP-event launch times and its algorithmic bytes;
+- converged-image RMSE of the 1024-spp image against an FP64-film oracle
+  image of 65,536 spp (independent seed), with the oracle image's own noise
+  floor;
+- closest hit on the same 1M-triangle mesh: 1920x1080 coherent primary rays
+  and 16,777,216 incoherent rays (rays/s, algorithmic fetch GB/s);
+- the 256x256, 16-spp, depth-5 box + 20,480-triangle icosphere workload;
+- the guided vs plain time to equal RMSE on the slit-lit scene;
+- the 64-scene batch;
+- cpu_baseline: the oracle (the C++ restatement of the reference's OpenMP CPU
+  path) on the host cores: a viewport slice of the 1024-spp workload (the
+  value's unit), the full 256x256 box image, the coherent frame and 1M of the
+  incoherent rays, with omp_get_max_threads(), nproc and the CPU model.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -33,48 +45,93 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sc, target_s=10.0):
-    """Oracle (reference semantics: recursive traversal, heap nodes, virtual
-    primitives, OpenMP dynamic rows) on a viewport slice of the same workload."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines(sc3, c1, mesh, coherent_rays, incoherent_rays):
+    """The oracle (reference semantics: recursive traversal, heap nodes,
+    virtual primitives, OpenMP dynamic rows / dynamic ray chunks) on the host
+    cores, each workload sized to a few seconds:
+    - the 1024-spp workload: an 80x60 centre viewport (the reference's own
+      viewport mechanism), samples sized to ~8 s;
+    - the box + icosphere workload in full (256x256, 16 spp, depth 5);
+    - closest hit on the 1M-triangle mesh: the 1920x1080 coherent frame and a
+      seeded 1,048,576-ray subset of the incoherent rays."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     import ctypes as C
 
-    threads = os.cpu_count() or 1
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = O.omp_max_threads()
+    out = {"cores": threads, "nproc": os.cpu_count(), "omp_max_threads": omp, "cpu_model": _cpu_model(),
+           "kind": "port"}
+
+    # 1024-spp workload, viewport slice
     t0 = time.perf_counter()
-    orc = O.Scene(sc.verts, sc.normals, sc.mat_ids, materials=sc.materials)
+    orc = O.Scene(sc3.verts, sc3.normals, sc3.mat_ids, materials=sc3.materials)
     build_s = time.perf_counter() - t0
-    cam = sc.camera
+    cam = sc3.camera
     vw, vh = 80, 60
     vp = (cam.height // 2 - vh // 2, cam.width // 2 - vw // 2, vw, vh)
-    p = sc.integrator.params()
+    p = sc3.integrator.params()
     p.viewport = (C.c_uint32 * 4)(*vp)
-    # probe with 1 sample per pixel, then size the sample to ~target_s
     p.spp_begin, p.spp_count = 0, 1
     t0 = time.perf_counter()
     orc.render(cam, p, threads=threads)
     probe = time.perf_counter() - t0
-    spp = int(max(1, min(1024, target_s / max(probe, 1e-3))))
+    spp = int(max(1, min(1024, 8.0 / max(probe, 1e-3))))
     p.spp_count = spp
     t0 = time.perf_counter()
     orc.render(cam, p, threads=threads)
     dt = time.perf_counter() - t0
     paths = vw * vh * spp
     orc.close()
-    return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"{vw}x{vh} centre viewport x {spp} spp = {paths} paths in {dt:.2f} s "
-                      f"(tree build {build_s:.1f} s not timed)"}
+    out.update({"value": paths / dt / 1e6, "unit": "Mpaths/s",
+                "sample": f"{vw}x{vh} centre viewport of the 800x600 1024-spp workload x {spp} spp = {paths} paths "
+                          f"in {dt:.2f} s (tree build {build_s:.1f} s not timed)"})
+
+    # box + icosphere, full image
+    orc = O.Scene(c1.verts, c1.normals, c1.mat_ids, materials=c1.materials)
+    t0 = time.perf_counter()
+    orc.render(c1.camera, c1.integrator.params(), threads=threads)
+    dt = time.perf_counter() - t0
+    n1 = c1.camera.width * c1.camera.height * c1.integrator.effective_spp()
+    orc.close()
+    out["box_icosphere_full_image"] = {"mpaths_s": round(n1 / dt / 1e6, 4), "paths": n1, "s": round(dt, 3)}
+
+    # closest hit, 1M-triangle mesh
+    t0 = time.perf_counter()
+    orc = O.Scene(mesh.verts, mesh.normals, mesh.mat_ids)
+    build_s = time.perf_counter() - t0
+    for name, rays in (("coherent_1920x1080", coherent_rays), ("incoherent_1M_subset", incoherent_rays)):
+        if len(rays) == 0:
+            continue
+        t0 = time.perf_counter()
+        orc.trace(rays, threads=threads)
+        dt = time.perf_counter() - t0
+        out[name] = {"mrays_s": round(len(rays) / dt / 1e6, 4), "rays": int(len(rays)), "s": round(dt, 3)}
+    out["mesh_tree_build_s_not_timed"] = round(build_s, 2)
+    orc.close()
+    return out
 
 
-def closest_hit_bench(dev, mesh, n_rays, reps=5):
-    from swpathtracer_amd import api, scenes
+def closest_hit_bench(dev, rays, reps=5):
+    """Closest hit of a resident ray batch on the device's current scene:
+    rays/s over `reps` launches (HIP events on the context stream) and the
+    algorithmic fetch rate (rays and hits plus the tree data and triangle
+    records each ray fetched, counted by one FETCH launch on the same rays)."""
     import swpathtracer_amd._lib as L
 
-    bvh = api.Bvh(mesh.verts, n_threads=os.cpu_count() or 8)
-    dev.set_scene(bvh, mesh.normals, mesh.mat_ids)
-    rays = scenes.incoherent_rays(mesh.verts, n_rays, seed=3)
+    n_rays = len(rays)
     d_rays = dev.to_device(rays)
-    del rays
     d_hits = dev.alloc(8 * n_rays)
     d_nodes = dev.alloc(4 * n_rays)
     d_tris = dev.alloc(4 * n_rays)
@@ -92,9 +149,45 @@ def closest_hit_bench(dev, mesh, n_rays, reps=5):
     tris = d_tris.download(np.uint32, n_rays).astype(np.float64)
     hits = d_hits.download(np.uint32, (n_rays, 2))
     byts = 32 * n_rays + 8 * n_rays + 32 * nodes.sum() + 48 * tris.sum()  # tree data in 32-B units
+    for b in (d_rays, d_hits, d_nodes, d_tris):
+        b.free()
     return {"rays": n_rays, "ms": ms, "grays_per_s": n_rays / (ms * 1e-3) / 1e9,
             "mean_tree_units_32B": nodes.mean(), "mean_tris": tris.mean(), "hit_frac": float((hits[:, 1] != L.SPT_MISS).mean()),
             "fetch_gbs": byts / (ms * 1e-3) / 1e9}
+
+
+def c3_converged_rmse(dev, sc, img1024, oracle_spp):
+    """Per-pixel RMSE of the 1024-spp image against an FP64-film image of
+    oracle_spp samples per pixel with an independent seed, rendered in
+    256-spp calls. The oracle image carries its own noise: for independent
+    estimates, E[RMSE^2] = noise(1024)^2 + noise(oracle_spp)^2, so the floor
+    the reference image adds is rmse * sqrt(1024 / oracle_spp) to first order."""
+    import swpathtracer_amd._lib as L
+
+    n = int(round(np.sqrt(oracle_spp)))
+    assert n * n == oracle_spp, "the FP64 oracle image needs a square jittered spp"
+    cam, integ = sc.camera, sc.integrator
+    npix = cam.width * cam.height
+    film64 = dev.alloc(npix * 3 * 8)
+    film64.zero()
+    t0 = time.perf_counter()
+    for b in range(0, oracle_spp, 256):
+        p = integ.params()
+        p.spp, p.seed = oracle_spp, 0x0FAC1E5
+        p.spp_begin, p.spp_count = b, min(256, oracle_spp - b)
+        p.flags = L.SPT_RENDER_FILM64
+        dev.render(cam, p, None, film64, None)
+    dev.sync()
+    ref = film64.download(np.float64, (cam.height, cam.width, 3)) / oracle_spp
+    film64.free()
+    dt = time.perf_counter() - t0
+    rmse = _rmse(img1024, ref)
+    floor = rmse * np.sqrt(1024.0 / oracle_spp)
+    log(f"converged image: RMSE {rmse:.6f} vs {oracle_spp}-spp FP64 image ({dt:.1f} s), floor {floor:.6f}")
+    return {"oracle_spp": oracle_spp, "oracle_film": "FP64", "oracle_seed": "independent", "rmse_1024spp": round(rmse, 6),
+            "oracle_noise_floor": round(float(floor), 6), "oracle_mean": [round(float(x), 5) for x in ref.reshape(-1, 3).mean(0)],
+            "image_mean": [round(float(x), 5) for x in img1024.reshape(-1, 3).mean(0)],
+            "oracle_render_s": round(dt, 1)}
 
 
 def _rmse(img, ref):
@@ -123,8 +216,10 @@ def guide_experiment(args):
     cam, integ = sc.camera, sc.integrator
     npix = W * H
 
-    # reference image: FP64 film, independent seed
+    # reference image: FP64 film, independent seed, a square jittered spp so
+    # that every stratum of every pixel is sampled equally
     osp = args.oracle_spp
+    assert int(round(np.sqrt(osp))) ** 2 == osp, "--oracle-spp must be a square"
     film64 = dev.alloc(npix * 3 * 8)
     film64.zero()
     t0 = time.perf_counter()
@@ -287,7 +382,10 @@ def main():
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--guide-full", action="store_true", help="keep rendering after equal RMSE (full curve)")
     ap.add_argument("--guide-res", type=int, nargs=2, default=(800, 600))
-    ap.add_argument("--oracle-spp", type=int, default=8192, help="spp of the FP64-film reference image")
+    ap.add_argument("--oracle-spp", type=int, default=16384,
+                    help="spp of the slit scene's FP64-film reference image (a square)")
+    ap.add_argument("--c3-oracle-spp", type=int, default=65536,
+                    help="spp of the 1024-spp workload's FP64-film oracle image (a square; 0: skip)")
     ap.add_argument("--guide-lr", type=float, default=1e-2)
     ap.add_argument("--guide-warmup", type=int, default=4, help="training passes before the counted image")
     ap.add_argument("--guide-warmup-spp", type=int, default=4, help="spp of each warm-up training pass")
@@ -377,7 +475,9 @@ def main():
     paths = args.steps * sps * npix
     log(f"timed {args.steps} steps: {wall:.3f} s, {paths / wall / 1e6:.1f} Mpaths/s, "
         f"{rays / wall / 1e9:.3f} Grays/s extension rays, extend kernel {ext_ms / 1e3:.3f} s")
-    # algorithmic bytes of the extension kernel: one instrumented step
+    img = film.download(np.float32, (cam.height, cam.width, 3)) / (args.steps * sps)
+    # algorithmic bytes of the extension kernel: one instrumented step (into
+    # the same film, after the image of the timed steps was read)
     step(0, L.SPT_RENDER_FILM32 | L.SPT_RENDER_FETCH_STATS)
     dev.sync()
     fs = dev.render_stats()
@@ -386,7 +486,6 @@ def main():
     # per extension ray: queue index 4 + origin 16 + direction 16 in, hit 16 + sort key 4 out
     bytes_per_ray = 56 + 32 * per_ray_nodes + 48 * per_ray_tris  # tree data counted in 32-B units
     achieved = bytes_per_ray * rays / (ext_ms * 1e-3) / 1e9
-    img = film.download(np.float32, (cam.height, cam.width, 3)) / (args.steps * sps)
 
     # HBM bytes per extension launch of this workload, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated)
@@ -399,11 +498,44 @@ def main():
             traffic = tj["hbm_bytes_per_launch"]
 
     extra = {}
+    if args.c3_oracle_spp and args.steps * sps == spp_total:  # the timed image is the full 1024-spp estimate
+        extra["converged_image"] = c3_converged_rmse(dev, sc, img, args.c3_oracle_spp)
+
+    # the box + 20,480-triangle icosphere workload: 256x256, 16 spp, depth 5
+    c1 = scenes.cornell_icosphere()
+    dev.set_scene(api.Bvh(c1.verts, n_threads=os.cpu_count() or 8), c1.normals, c1.mat_ids)
+    dev.set_materials(c1.materials)
+    c1_film = dev.alloc(c1.camera.width * c1.camera.height * 12)
+    p1 = c1.integrator.params()
+    p1.flags = L.SPT_RENDER_FILM32
+    dev.render(c1.camera, p1, c1_film, None, None)
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dev.render(c1.camera, p1, c1_film, None, None)
+    dev.sync()
+    c1_s = (time.perf_counter() - t0) / 5
+    c1_paths = c1.camera.width * c1.camera.height * c1.integrator.effective_spp()
+    extra["box_icosphere_256x256"] = {"triangles": int(c1.n_tris), "spp": c1.integrator.effective_spp(),
+                                      "max_depth": 5, "paths": c1_paths, "ms": round(c1_s * 1e3, 3),
+                                      "mpaths_s": round(c1_paths / c1_s / 1e6, 2)}
+    log(f"box + icosphere: {extra['box_icosphere_256x256']}")
+    c1_film.free()
+
+    mesh = coherent = incoherent = None
     if not args.no_closest_hit:
         mesh = scenes.displaced_mesh(args.rings, args.segments)
-        ch = closest_hit_bench(dev, mesh, 16_777_216)
-        log(f"closest hit: {ch}")
+        dev.set_scene(api.Bvh(mesh.verts, n_threads=os.cpu_count() or 8), mesh.normals, mesh.mat_ids)
+        cam2 = api.pinhole(1920, 1080, position=(0, 0, 3), look_at=(0, 0, 0), fov=40.0)
+        coherent = dev.camera_rays(cam2)
+        ch = closest_hit_bench(dev, coherent)
+        log(f"closest hit, coherent: {ch}")
+        extra["closest_hit_coherent"] = {k: round(v, 4) if isinstance(v, float) else v for k, v in ch.items()}
+        incoherent = scenes.incoherent_rays(mesh.verts, 16_777_216, seed=3)
+        ch = closest_hit_bench(dev, incoherent)
+        log(f"closest hit, incoherent: {ch}")
         extra["closest_hit_incoherent"] = {k: round(v, 4) if isinstance(v, float) else v for k, v in ch.items()}
+        incoherent = incoherent[np.sort(np.random.default_rng(21).choice(len(incoherent), 1 << 20, replace=False))]
     film.free()
     dev.close()  # each experiment below opens its own context
     if not args.no_guide:
@@ -412,7 +544,11 @@ def main():
         extra["batch"] = batch_bench(args)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sc)
+        if mesh is None:
+            mesh = scenes.displaced_mesh(args.rings, args.segments)
+            coherent = np.zeros((0, 8), np.float32)
+            incoherent = scenes.incoherent_rays(mesh.verts, 1 << 20, seed=3)
+        cpu = cpu_baselines(sc, c1, mesh, coherent, incoherent)
         log(f"cpu baseline: {cpu}")
 
     value = paths / wall / 1e6
@@ -445,6 +581,8 @@ def main():
         "cpu_baseline": cpu,
         "image_mean": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)],
     }
+    if cpu:
+        out["cpu_baseline"]["value"] = round(cpu["value"], 4)
     out.update(extra)
     print(json.dumps(out), flush=True)
 

@@ -1326,3 +1326,5 @@ int orc_raycast(const orc_scene* s, const spt_camera* cam, uint32_t kind, void* 
 }
 
 }  // extern "C"
+
+extern "C" int orc_omp_max_threads(void) { return omp_get_max_threads(); }

@@ -79,6 +79,10 @@ float orc_schlick_ior(float ior_i, float ior_t, float cos_theta);
 int orc_render(const orc_scene* s, const spt_camera* cam, const spt_render_params* p, float* rgb32, double* rgb64,
                float* path_rgb, uint64_t* overflows, int n_threads);
 
+/* omp_get_max_threads() of the library (the thread count the reference's
+ * render loop reports, integrators/pathtracer.cpp:74). */
+int orc_omp_max_threads(void);
+
 /* Ray casters (integrators/raycaster.h and the three ray caster sources). */
 int orc_raycast(const orc_scene* s, const spt_camera* cam, uint32_t kind, void* out, int n_threads);
 

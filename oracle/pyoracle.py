@@ -51,6 +51,7 @@ def load():
         "orc_schlick_ior": (f, [f, f, f]),
         "orc_render": (i, [vp, C.POINTER(L.Camera), C.POINTER(L.RenderParams), vp, vp, vp, vp, i]),
         "orc_raycast": (i, [vp, C.POINTER(L.Camera), u32, vp, i]),
+        "orc_omp_max_threads": (i, []),
     }
     for name, (res, args) in protos.items():
         fn = getattr(lib, name)
@@ -241,3 +242,8 @@ def layer_eval(layer, wo, wi):
     b = np.ascontiguousarray(wi, np.float32)
     assert load().orc_layer_eval(C.byref(layer), _p(a), _p(b), _p(out)) == 0
     return out
+
+
+def omp_max_threads() -> int:
+    """omp_get_max_threads() inside the oracle library."""
+    return int(load().orc_omp_max_threads())

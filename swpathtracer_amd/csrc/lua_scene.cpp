@@ -198,7 +198,7 @@ struct Table {
                 for (;;) {
                     Value next = Value::integer((double)arr.size() + 1.0);
                     auto it = index.find(make_key(next));
-                    if (it == index.end()) break;
+                    if (it == index.end() || entries[it->second].second.is_nil()) break;
                     arr.push_back(entries[it->second].second);
                     erase_hash(next);
                 }
@@ -219,12 +219,16 @@ struct Table {
         }
     }
     void set(const std::string& name, const Value& val) { set(Value::string(name), val); }
+    // A removed hash entry keeps its key and position with a nil value, so
+    // a traversal (next) that clears fields as it goes can continue from it.
     void erase_hash(const Value& key) {
         auto it = index.find(make_key(key));
         if (it == index.end()) return;
-        entries[it->second].first = Value();
         entries[it->second].second = Value();
-        index.erase(it);
+    }
+    bool has_hash(const Value& key) const {
+        auto it = index.find(make_key(key));
+        return it != index.end() && !entries[it->second].second.is_nil();
     }
     // The border # returns: the array part's length, extended through the
     // hash part while consecutive integer keys are present.
@@ -232,7 +236,7 @@ struct Table {
         size_t n = arr.size();
         while (n > 0 && arr[n - 1].is_nil()) --n;
         if (n == arr.size())
-            while (index.count(make_key(Value::integer((double)n + 1.0)))) ++n;
+            while (has_hash(Value::integer((double)n + 1.0))) ++n;
         return n;
     }
     // Iteration order of next/pairs: the array part, then the hash part in
@@ -241,12 +245,15 @@ struct Table {
         size_t pos = 0;  // position in the combined sequence to start from
         if (!key.is_nil()) {
             size_t i;
+            auto it = index.find(make_key(key));
             if (array_index(key, i) && i < arr.size()) {
                 pos = i + 1;
-            } else {
-                auto it = index.find(make_key(key));
-                if (it == index.end()) throw std::runtime_error("invalid key to 'next'");
+            } else if (it != index.end()) {
                 pos = arr.size() + it->second + 1;
+            } else if (array_index(key, i)) {
+                pos = arr.size();  // the array part shrank under the traversal
+            } else {
+                throw std::runtime_error("invalid key to 'next'");
             }
         }
         for (; pos < arr.size(); ++pos)
@@ -256,7 +263,7 @@ struct Table {
                 return true;
             }
         for (size_t e = pos - arr.size(); e < entries.size(); ++e)
-            if (!entries[e].first.is_nil()) {
+            if (!entries[e].second.is_nil()) {
                 k_out = entries[e].first;
                 v_out = entries[e].second;
                 return true;

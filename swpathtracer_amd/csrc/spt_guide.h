@@ -1,6 +1,7 @@
 // spt_guide.h - learned path guide on the device: input encoding, a 32-64-64-
-// 64-40 MLP evaluated 16 rows per wave with bf16 MFMA (v_mfma_f32_16x16x16_bf16,
-// fp32 accumulation), the 8-lobe von Mises-Fisher mixture it outputs (density,
+// 64-40 MLP evaluated 16 rows per wave with gfx950 bf16 MFMA
+// (v_mfma_f32_16x16x32_bf16, fp32 accumulation), the 8-lobe von Mises-Fisher
+// mixture it outputs (density,
 // sampling, KL-loss gradient) and the fused backward pass.
 //
 // Parameter vector (fp32 master, 13,552 floats): W1 64x32 | b1 64 | W2 64x64 |
@@ -22,6 +23,8 @@ constexpr int kRecord = 24;  // floats per training record: p n wo wi W g rad_af
 constexpr float kKappaMin = 1e-2f, kKappaMax = 1e4f;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8s __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // round-to-nearest-even float -> bfloat16 bits, and back
@@ -109,10 +112,16 @@ __device__ inline bf16x4 ld4s(const unsigned short* p, int stride) {
     return r;
 }
 
-// v_mfma_f32_16x16x16_bf16 lane layout (wave64), lane l, r = l % 16, g = l / 16:
+// Operand layout of the data (wave64), lane l, r = l % 16, g = l / 16, per
+// 16-wide k-tile:
 //   A (16 rows x 16 k):  A[r][4g + j]         j = 0..3
 //   B (16 k x 16 cols):  B[4g + j][r]
 //   D (16 x 16, f32):    D[4g + i][r]         i = 0..3
+// v_mfma_f32_16x16x32_bf16 takes 8 bf16 of A and of B per lane: element e of
+// lane (r, g) multiplies element e of lane (c, g) for every e, and all 32
+// products per (row, col) are summed. k is only a summation index, so two
+// k-tiles are fed as one by concatenating the lane's 4 values of tile k and
+// of tile k + 1 in A and in B alike (mfma_k32).
 // ---------------------------------------------------------------------------
 // Register-resident forward pass ("column" form): the layer is computed as
 // out^T = W * in^T, weights as the MFMA A operand (from LDS) and 16 samples as
@@ -133,6 +142,18 @@ __device__ inline bf16x4 pack_relu(f32x4 v) {
 
 __device__ inline f32x4 bias4(const float* b) { return f32x4{b[0], b[1], b[2], b[3]}; }
 
+// D += A B over two 16-wide k-tiles (a0, b0: tile k; a1, b1: tile k + 1).
+__device__ inline f32x4 mfma_k32(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b1, f32x4 c) {
+    const bf16x8s a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+    const bf16x8s b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+// D += A B over one 16-wide k-tile (the odd tile of a 48-wide reduction).
+__device__ inline f32x4 mfma_k16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
 // x[kt]: features 16 kt + 4 g + j of sample r (bf16); y[t]: raw outputs
 // 16 t + 4 g + i of sample r (f32), t = 0..2 (outputs 40..47 are padding).
 __device__ inline void mlp_cols_fwd(const SharedWeights& s, const bf16x4 x[2], f32x4 y[3]) {
@@ -142,36 +163,36 @@ __device__ inline void mlp_cols_fwd(const SharedWeights& s, const bf16x4 x[2], f
     for (int t = 0; t < 4; ++t) {
         f32x4 acc = bias4(s.b1 + 16 * t + 4 * g);
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(s.w1 + (16 * t + r) * kEnc + 16 * k + 4 * g), x[k], acc,
-                                                            0, 0, 0);
+        for (int k = 0; k < 2; k += 2)
+            acc = mfma_k32(ld4(s.w1 + (16 * t + r) * kEnc + 16 * k + 4 * g),
+                           ld4(s.w1 + (16 * t + r) * kEnc + 16 * (k + 1) + 4 * g), x[k], x[k + 1], acc);
         ha[t] = pack_relu(acc);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         f32x4 acc = bias4(s.b2 + 16 * t + 4 * g);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(s.w2 + (16 * t + r) * kHid + 16 * k + 4 * g), ha[k], acc,
-                                                            0, 0, 0);
+        for (int k = 0; k < 4; k += 2)
+            acc = mfma_k32(ld4(s.w2 + (16 * t + r) * kHid + 16 * k + 4 * g),
+                           ld4(s.w2 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), ha[k], ha[k + 1], acc);
         hb[t] = pack_relu(acc);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         f32x4 acc = bias4(s.b3 + 16 * t + 4 * g);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(s.w3 + (16 * t + r) * kHid + 16 * k + 4 * g), hb[k], acc,
-                                                            0, 0, 0);
+        for (int k = 0; k < 4; k += 2)
+            acc = mfma_k32(ld4(s.w3 + (16 * t + r) * kHid + 16 * k + 4 * g),
+                           ld4(s.w3 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), hb[k], hb[k + 1], acc);
         ha[t] = pack_relu(acc);
     }
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
         f32x4 acc = bias4(s.b4 + 16 * t + 4 * g);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(s.w4 + (16 * t + r) * kHid + 16 * k + 4 * g), ha[k], acc,
-                                                            0, 0, 0);
+        for (int k = 0; k < 4; k += 2)
+            acc = mfma_k32(ld4(s.w4 + (16 * t + r) * kHid + 16 * k + 4 * g),
+                           ld4(s.w4 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), ha[k], ha[k + 1], acc);
         y[t] = acc;
     }
 }

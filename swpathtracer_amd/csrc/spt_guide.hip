@@ -147,9 +147,9 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
         for (int t = 0; t < 4; ++t) {
             f32x4 acc = bias4(sw.b1 + 16 * t + 4 * g);
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw.w1 + (16 * t + r) * kEnc + 16 * k + 4 * g), x[k],
-                                                                acc, 0, 0, 0);
+            for (int k = 0; k < 2; k += 2)
+                acc = mfma_k32(ld4(sw.w1 + (16 * t + r) * kEnc + 16 * k + 4 * g),
+                               ld4(sw.w1 + (16 * t + r) * kEnc + 16 * (k + 1) + 4 * g), x[k], x[k + 1], acc);
             ha[t] = pack_relu(acc);
             if (keep) st_col4(ts.h1T, ts.rows, 16 * t + 4 * g, sq, ha[t]);
         }
@@ -157,9 +157,9 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
         for (int t = 0; t < 4; ++t) {
             f32x4 acc = bias4(sw.b2 + 16 * t + 4 * g);
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw.w2 + (16 * t + r) * kHid + 16 * k + 4 * g), ha[k],
-                                                                acc, 0, 0, 0);
+            for (int k = 0; k < 4; k += 2)
+                acc = mfma_k32(ld4(sw.w2 + (16 * t + r) * kHid + 16 * k + 4 * g),
+                               ld4(sw.w2 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), ha[k], ha[k + 1], acc);
             hb[t] = pack_relu(acc);
             if (keep) st_col4(ts.h2T, ts.rows, 16 * t + 4 * g, sq, hb[t]);
         }
@@ -167,9 +167,9 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
         for (int t = 0; t < 4; ++t) {
             f32x4 acc = bias4(sw.b3 + 16 * t + 4 * g);
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw.w3 + (16 * t + r) * kHid + 16 * k + 4 * g), hb[k],
-                                                                acc, 0, 0, 0);
+            for (int k = 0; k < 4; k += 2)
+                acc = mfma_k32(ld4(sw.w3 + (16 * t + r) * kHid + 16 * k + 4 * g),
+                               ld4(sw.w3 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), hb[k], hb[k + 1], acc);
             ha[t] = pack_relu(acc);
             if (keep) st_col4(ts.h3T, ts.rows, 16 * t + 4 * g, sq, ha[t]);
         }
@@ -177,9 +177,9 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
         for (int t = 0; t < 3; ++t) {
             f32x4 acc = bias4(sw.b4 + 16 * t + 4 * g);
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw.w4 + (16 * t + r) * kHid + 16 * k + 4 * g), ha[k],
-                                                                acc, 0, 0, 0);
+            for (int k = 0; k < 4; k += 2)
+                acc = mfma_k32(ld4(sw.w4 + (16 * t + r) * kHid + 16 * k + 4 * g),
+                               ld4(sw.w4 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), ha[k], ha[k + 1], acc);
             if (16 * t + 4 * g < kOut)
                 for (int i = 0; i < 4; ++i) ys[(16 * q + r) * kOut + 16 * t + 4 * g + i] = acc[i];
         }
@@ -227,9 +227,8 @@ __global__ __launch_bounds__(kBlock) void k_train_bwd(const float* __restrict__ 
 #pragma unroll
         for (int t = 0; t < 4; ++t) {  // dz3: K = 48 outputs
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_wT(sw.w4, kHid, k, t), dy[k], acc, 0, 0, 0);
+            acc = mfma_k32(ld_wT(sw.w4, kHid, 0, t), ld_wT(sw.w4, kHid, 1, t), dy[0], dy[1], acc);
+            acc = mfma_k16(ld_wT(sw.w4, kHid, 2, t), dy[2], acc);
             const bf16x4 h = ld_col4(ts.h3T, ts.rows, 16 * t + 4 * g, s);
             for (int i = 0; i < 4; ++i)
                 if (bf2f((unsigned short)h[i]) <= 0.0f) acc[i] = 0.0f;
@@ -240,8 +239,8 @@ __global__ __launch_bounds__(kBlock) void k_train_bwd(const float* __restrict__ 
         for (int t = 0; t < 4; ++t) {  // dz2
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_wT(sw.w3, kHid, k, t), za[k], acc, 0, 0, 0);
+            for (int k = 0; k < 4; k += 2)
+                acc = mfma_k32(ld_wT(sw.w3, kHid, k, t), ld_wT(sw.w3, kHid, k + 1, t), za[k], za[k + 1], acc);
             const bf16x4 h = ld_col4(ts.h2T, ts.rows, 16 * t + 4 * g, s);
             for (int i = 0; i < 4; ++i)
                 if (bf2f((unsigned short)h[i]) <= 0.0f) acc[i] = 0.0f;
@@ -252,8 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_train_bwd(const float* __restrict__ 
         for (int t = 0; t < 4; ++t) {  // dz1
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld_wT(sw.w2, kHid, k, t), zb[k], acc, 0, 0, 0);
+            for (int k = 0; k < 4; k += 2)
+                acc = mfma_k32(ld_wT(sw.w2, kHid, k, t), ld_wT(sw.w2, kHid, k + 1, t), zb[k], zb[k + 1], acc);
             const bf16x4 h = ld_col4(ts.h1T, ts.rows, 16 * t + 4 * g, s);
             for (int i = 0; i < 4; ++i)
                 if (bf2f((unsigned short)h[i]) <= 0.0f) acc[i] = 0.0f;
@@ -289,7 +288,8 @@ __global__ __launch_bounds__(kBlock) void k_train_wgrad(TrainScratch ts, uint32_
     f32x4 acc[4];
     for (int k = 0; k < 4; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float bsum = 0.0f;
-    for (uint32_t s0 = s_begin; s0 < s_end; s0 += 16) {
+    // dz of 4 samples s0 + 4 g .. + 3 for output row 16 ot + r (and their bias sum)
+    auto dz4 = [&](uint32_t s0) {
         bf16x4 a;
         if (layer == 3) {
             const float4 v = *reinterpret_cast<const float4*>(ts.dyT + (size_t)(16 * ot + r) * ts.rows + s0 + 4 * g);
@@ -300,11 +300,23 @@ __global__ __launch_bounds__(kBlock) void k_train_wgrad(TrainScratch ts, uint32_
             bsum += (bf2f((unsigned short)a[0]) + bf2f((unsigned short)a[1])) +
                     (bf2f((unsigned short)a[2]) + bf2f((unsigned short)a[3]));
         }
+        return a;
+    };
+    // samples are the reduction axis: 32 per MFMA, a last odd 16 alone
+    uint32_t s0 = s_begin;
+    for (; s0 + 32 <= s_end; s0 += 32) {
+        const bf16x4 a0 = dz4(s0), a1 = dz4(s0 + 16);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (k < n_kt)
-                acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ld_row4(in, ts.rows, 16 * k + r, s0 + 4 * g),
-                                                                   acc[k], 0, 0, 0);
+                acc[k] = mfma_k32(a0, a1, ld_row4(in, ts.rows, 16 * k + r, s0 + 4 * g),
+                                  ld_row4(in, ts.rows, 16 * k + r, s0 + 16 + 4 * g), acc[k]);
+    }
+    if (s0 < s_end) {
+        const bf16x4 a = dz4(s0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < n_kt) acc[k] = mfma_k16(a, ld_row4(in, ts.rows, 16 * k + r, s0 + 4 * g), acc[k]);
     }
     float* out = ts.partial + (size_t)split * kParams;
     const int wofs = layer == 0 ? kW1 : layer == 1 ? kW2 : layer == 2 ? kW3 : kW4;

@@ -1189,16 +1189,21 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     a.records = ctx->guide.d_records;
     a.rec_cap = ctx->guide.records_cap;
     // record about records_cap of this render's candidate vertices, estimated
-    // from the candidates per path of the previous training render; with
-    // in-call training about a quarter of the ring per training round (a
-    // bounce of a full queue meets about capacity / rays-per-path paths)
+    // from the candidates per path of the previous training render. With
+    // in-call training the ring must not wrap between the count a training
+    // round reads and the moment its steps run: the host runs up to
+    // check_every bounces ahead of the device, so one round spans
+    // train_every + check_every + 1 bounces, and that span writes at most a
+    // quarter of the ring (a bounce of a full queue meets about capacity /
+    // rays-per-path paths); the host skips batches older than half a ring
     a.rec_ring = (a.train && ctx->guide.train_every) ? 1u : 0u;
     if (ctx->guide.cand_per_path > 0.0 && ctx->guide.records_cap > 0) {
         double want = ctx->guide.cand_per_path * (double)total;
         if (a.rec_ring) {
+            const spt_tuning tn = resolve_tuning(ctx->tuning);
             const double per_bounce = ctx->guide.cand_per_path / ctx->guide.rays_per_path *
-                                      (double)std::min<uint64_t>(total, resolve_tuning(ctx->tuning).path_capacity);
-            want = per_bounce * ctx->guide.train_every * 4.0;
+                                      (double)std::min<uint64_t>(total, tn.path_capacity);
+            want = per_bounce * (double)(ctx->guide.train_every + tn.check_every + 1) * 4.0;
         }
         ctx->guide.rec_stride =
             (uint32_t)std::min<double>(1 << 30, std::max(1.0, std::ceil(want / ctx->guide.records_cap)));

@@ -66,7 +66,7 @@ quad({-a, -a,  a}, { a, -a,  a}, { a, -a, -a}, {-a, -a, -a}, white)  -- floor
 quad({-a,  a, -a}, { a,  a, -a}, { a,  a,  a}, {-a,  a,  a}, white)  -- ceiling
 quad({-a, -a,  a}, {-a, -a, -a}, {-a,  a, -a}, {-a,  a,  a}, red)    -- left
 quad({ a, -a, -a}, { a, -a,  a}, { a,  a,  a}, { a,  a, -a}, white)  -- right
-local l, y = 0.3, a - 0.001
+local l, y = 0.6, a - 0.001
 quad({-l, y, -l}, {l, y, -l}, {l, y, l}, {-l, y, l}, light)
 
 Sphere{ center = {-0.45, -0.6, -0.3}, radius = 0.4, material = copper }

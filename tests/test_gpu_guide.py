@@ -84,7 +84,9 @@ def test_eval_matches_oracle(device, box_scene, n):
     err = np.abs(y - yr) / (1.0 + np.abs(yr))
     print(f"n={n} max err {err.max():.3e} mean {err.mean():.3e} |y| {np.abs(yr).mean():.3f}")
     assert np.all(np.isfinite(y))
-    assert err.mean() < 2e-3 and err.max() < 5e-2
+    # measured on gfx950: mean 1e-7, max 3.6e-4 over 65,543 rows (bf16
+    # rounding of a hidden activation flipping at a tie)
+    assert err.mean() < 1e-6 and err.max() < 2e-3
     # a transposed operand or a wrong lane map breaks this by orders of magnitude
     assert np.corrcoef(y.ravel(), yr.ravel())[0, 1] > 0.9999
 
@@ -109,7 +111,9 @@ def test_gradient_matches_oracle(device, box_scene):
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
         print(f"{name}: rel {rel:.3e} cos {cos:.7f}")
-        assert rel < 2e-2 and cos > 0.9995, name
+        # measured: rel <= 2.8e-3, cos >= 0.999996 (bf16 activations and
+        # products, fp32 accumulation in a different order)
+        assert rel < 6e-3 and cos > 0.99995, name
     assert np.all(g[G._pad_mask()] == 0.0)
 
 

@@ -256,6 +256,10 @@ print(("abc"):upper(), string.sub("hello", 2, -2), string.rep("ab", 3))
 --[[ a long
 comment ]] print([[long
 string]])
+local tt = {a = 1, b = 2, c = 3, 10, 20}
+local cnt = 0
+for k, v in pairs(tt) do tt[k] = nil; cnt = cnt + 1 end
+print(cnt, next(tt))
 """
 
 EXPECTED = """3\t50\ta
@@ -276,6 +280,7 @@ ab12\ttrue\t15\t5
 ABC\tell\tababab
 long
 string
+5\tnil
 """
 
 
