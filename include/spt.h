@@ -298,6 +298,20 @@ enum {
 spt_status spt_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits, uint32_t mode,
                      uint32_t* d_tests, uint32_t* d_isects);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): spt_trace in SPT_TRACE_CLOSEST mode with no allocation
+ * at all. Its scratch (ray counter, overflow flag, traversal-stack spill) is
+ * a static device array of SPT_NOALLOC_SCRATCH_WORDS 32-bit words (32 MiB)
+ * per GPU, held until the process ends; the persistent grid is capped to what
+ * fits in it. Rejects n > SPT_NOALLOC_MAX_RAYS (SPT_ERR_INVALID) and trees
+ * whose stack does not fit one block (SPT_ERR_UNSUPPORTED). Calls that overlap
+ * on one GPU (several contexts or threads) race on the shared scratch and must
+ * be ordered by the caller. A stack overflow is reported by spt_sync. */
+#define SPT_NOALLOC_SCRATCH_WORDS (8u * 1024u * 1024u)
+#define SPT_NOALLOC_MAX_RAYS 0xFFFFFF00ull
+spt_status spt_trace_noalloc(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits);
+
 /* Primary rays: d_rays[y * w + x] = camera ray through image point
  * (x + sx, y + sy) with (sx, sy) = (0.5, 0.5), the single sample of
  * RegularPixelSampler(1) that the ray casters use (integrators/raycaster.h:45). */

@@ -57,6 +57,8 @@ SPT_TRACE_FETCH = 3
 SPT_RAYCAST_NORMAL = 0
 SPT_RAYCAST_DEPTH = 1
 SPT_RAYCAST_COUNT = 2
+SPT_NOALLOC_SCRATCH_WORDS = 8 * 1024 * 1024
+SPT_NOALLOC_MAX_RAYS = 0xFFFFFF00
 SPT_LUA_NONE = 0
 SPT_LUA_PATHTRACER = 1
 SPT_LUA_RAYCAST_NORMAL = 2
@@ -198,6 +200,7 @@ PROTOTYPES = {
     "spt_batch_get_stats": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(BatchStats), C.c_void_p]),
     "spt_batch_autotune": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(Tuning)]),
     "spt_batch_destroy": (None, [C.c_void_p]),
+    "spt_trace_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "spt_lua_load": (C.c_int32, [C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(C.c_void_p), C.c_char_p,
                                  C.c_size_t]),
     "spt_lua_get_info": (C.c_int32, [C.c_void_p, C.POINTER(LuaInfo)]),

@@ -403,6 +403,7 @@ spt_status spt_create(int device, spt_ctx** out) {
         return SPT_ERR_DEVICE;
     }
     std::memset(hs, 0, 256);
+    if (init_static_scratch(ctx) != SPT_OK) ctx->scratch = nullptr;  // only spt_trace_noalloc needs it
     for (hipEvent_t& e : ctx->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             spt_destroy(ctx);
@@ -626,6 +627,12 @@ spt_status spt_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit*
     if (mode == SPT_TRACE_FETCH && (!d_tests || !d_isects)) return SPT_ERR_INVALID;
     if (!ctx->has_scene) return set_error(ctx, SPT_ERR_STATE, "no scene uploaded");
     return launch_trace(ctx, d_rays, n, d_hits, mode, d_tests, d_isects);
+}
+
+spt_status spt_trace_noalloc(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits) {
+    if (!ctx || (n && (!d_rays || !d_hits))) return SPT_ERR_INVALID;
+    if (!ctx->has_scene) return set_error(ctx, SPT_ERR_STATE, "no scene uploaded");
+    return launch_trace_noalloc(ctx, d_rays, n, d_hits);
 }
 
 spt_status spt_camera_rays(spt_ctx* ctx, const spt_camera* cam, spt_ray32* d_rays) {

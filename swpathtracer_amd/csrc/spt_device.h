@@ -735,6 +735,7 @@ struct spt_ctx {
     // status word written by kernels (1 = stack overflow), then two 64-bit
     // fetch counters (nodes, triangles) at byte offset 16
     uint32_t* d_flags = nullptr;
+    uint32_t* scratch = nullptr;  // this device's static scratch (spt_trace_noalloc), never freed
     // HIP events: [0..1] bracket extend launches, [2..9] user slots (spt_event_*)
     hipEvent_t ev[10] = {};
     // event pairs bracketing the extend launches of one render call
@@ -788,6 +789,11 @@ spt_status ensure_spill(spt_ctx* ctx, size_t words);
 spt_status launch_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits, uint32_t mode,
                         uint32_t* d_tests, uint32_t* d_isects);
 spt_status launch_raycast(spt_ctx* ctx, const spt_camera* cam, uint32_t kind, void* d_out);
+// The allocation-free closest-hit path (spt_trace_noalloc) and the lookup of
+// its static scratch on the context's device (from spt_create).
+constexpr int kMaxScratchDevices = 64;
+spt_status init_static_scratch(spt_ctx* ctx);
+spt_status launch_trace_noalloc(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits);
 spt_status launch_camera_rays(spt_ctx* ctx, const spt_camera* cam, spt_ray32* d_rays);
 GuideBox guide_box(const spt_ctx* ctx);
 spt_status guide_eval_launch(spt_ctx* ctx, const float* d_q, uint32_t stride, uint32_t n, float* d_out);

@@ -1,6 +1,8 @@
 // spt_trace.hip - closest-hit ray casting kernels (batched
 // AccelStructure::intersect, acceleration/accel_structure.h:29-30) and the
 // three ray casters (integrators/raycaster.h:61-99).
+#include <mutex>
+
 #include "spt_device.h"
 
 namespace spt {
@@ -109,6 +111,22 @@ __global__ __launch_bounds__(kBlock) void k_camera_rays(DevCamera cam, float4* _
     rays[2 * i + 1] = make_float4(d.x, d.y, d.z, 0.0f);
 }
 
+// Scratch of the allocation-free closest-hit entry point
+// (spt_trace_noalloc): words [0, 16) hold the overflow flag (word 0) and the
+// persistent kernel's ray counter (word 8), words [64, ...) the traversal
+// stacks' spill area. Each device holds its own copy; host code reaches it
+// only through the address hipGetSymbolAddress returns for that device.
+static __device__ __align__(256) uint32_t scratch[SPT_NOALLOC_SCRATCH_WORDS];
+constexpr uint32_t kScratchSpill = 64;
+
+__global__ void k_scratch_reset(uint32_t* __restrict__ ctl) {
+    if (threadIdx.x < kScratchSpill) ctl[threadIdx.x] = 0u;
+}
+
+__global__ void k_scratch_fold_overflow(const uint32_t* __restrict__ ctl, uint32_t* __restrict__ ctx_flags) {
+    if (threadIdx.x == 0 && ctl[0] != 0u) atomicOr(ctx_flags, 1u);
+}
+
 uint32_t grid_for(const spt_ctx* ctx, uint64_t n) {
     uint64_t want = (n + kBlock - 1) / kBlock;
     uint64_t cap = (uint64_t)ctx->num_cus * 16;  // resident-size grid, stride loop beyond
@@ -172,6 +190,61 @@ spt_status launch_trace(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_h
         hipLaunchKernelGGL((k_trace<false, false, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc, rays, n,
                            hits, nullptr, nullptr, ctx->d_spill, ctx->d_flags);
     return check_hip(ctx, hipGetLastError(), "trace launch");
+}
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+bool g_scratch_done[kMaxScratchDevices] = {};
+}  // namespace
+
+spt_status init_static_scratch(spt_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxScratchDevices) {
+        ctx->scratch = nullptr;
+        return SPT_OK;  // the allocation-free entry point reports it when called
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[ctx->device]) {
+        // the calling thread has ctx->device current (spt_create set it)
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->scratch = nullptr;
+            return check_hip(ctx, e, "static scratch address");
+        }
+        g_scratch_ptr[ctx->device] = ptr;
+        g_scratch_done[ctx->device] = true;
+    }
+    ctx->scratch = static_cast<uint32_t*>(g_scratch_ptr[ctx->device]);
+    return SPT_OK;
+}
+
+spt_status launch_trace_noalloc(spt_ctx* ctx, const spt_ray32* d_rays, uint64_t n, spt_hit* d_hits) {
+    if (n > SPT_NOALLOC_MAX_RAYS)
+        return set_error(ctx, SPT_ERR_INVALID, "spt_trace_noalloc: more than SPT_NOALLOC_MAX_RAYS rays");
+    if (!ctx->scratch) return set_error(ctx, SPT_ERR_STATE, "spt_trace_noalloc: no static scratch on this device");
+    if (n == 0) return SPT_OK;
+    const void* kern = reinterpret_cast<const void*>(&k_trace_p<false>);
+    // spill words per lane: the stack entries beyond the LDS part
+    const uint64_t per_lane = ctx->stack_cap4 > kLdsStack ? ctx->stack_cap4 - kLdsStack : 1;
+    const uint64_t room = (uint64_t)SPT_NOALLOC_SCRATCH_WORDS - kScratchSpill;
+    const uint64_t max_blocks = room / (per_lane * kBlock);
+    if (max_blocks == 0)
+        return set_error(ctx, SPT_ERR_UNSUPPORTED, "spt_trace_noalloc: the tree is too deep for the static scratch");
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(persistent_grid(ctx, kern), max_blocks);
+    uint32_t* ctl = ctx->scratch;
+    uint32_t* spill = ctx->scratch + kScratchSpill;
+    // a fill kernel, not a memset node: callers may capture this in a graph
+    hipLaunchKernelGGL(k_scratch_reset, dim3(1), dim3(kScratchSpill), 0, ctx->stream, ctl);
+    const uint32_t want = ctx->tuning.refill_lanes ? ctx->tuning.refill_lanes : 16u;
+    const uint32_t refill = std::min<uint32_t>(64, std::max<uint32_t>(1, want));
+    hipLaunchKernelGGL(k_trace_p<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->scene(),
+                       reinterpret_cast<const float4*>(d_rays), (uint32_t)n, reinterpret_cast<uint2*>(d_hits), nullptr,
+                       nullptr, spill, ctl, refill);
+    // a stack overflow is reported by spt_sync, as for spt_trace
+    hipLaunchKernelGGL(k_scratch_fold_overflow, dim3(1), dim3(64), 0, ctx->stream, ctl, ctx->d_flags);
+    return check_hip(ctx, hipGetLastError(), "trace (no allocation) launch");
 }
 
 spt_status launch_camera_rays(spt_ctx* ctx, const spt_camera* cam, spt_ray32* d_rays) {
