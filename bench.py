@@ -382,8 +382,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--guide-full", action="store_true", help="keep rendering after equal RMSE (full curve)")
     ap.add_argument("--guide-res", type=int, nargs=2, default=(800, 600))
-    ap.add_argument("--oracle-spp", type=int, default=16384,
-                    help="spp of the slit scene's FP64-film reference image (a square)")
+    ap.add_argument("--oracle-spp", type=int, default=65536,
+                    help="spp of the slit scene's FP64-film oracle image (a square)")
     ap.add_argument("--c3-oracle-spp", type=int, default=65536,
                     help="spp of the 1024-spp workload's FP64-film oracle image (a square; 0: skip)")
     ap.add_argument("--guide-lr", type=float, default=1e-2)

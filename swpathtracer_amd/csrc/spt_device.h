@@ -309,14 +309,19 @@ __device__ inline float qbyte(uint32_t w, int b) { return (float)((w >> (8 * b))
 // q * scale, t = (origin + q * scale - o) / d = q * (scale / d) + (origin / d
 // - o / d), one FMA per plane with a = scale * inv and b = origin * inv + oi
 // per node and axis. Rounding differs from the world-space form by a few
-// ulps of the same terms, far inside the 1e-3 box padding.
+// ulps of the same terms, far inside the 1e-3 box padding. The planes come
+// in near/far order for the ray: t = q * a + b is monotonic in q, so for
+// a >= 0 (a has the sign of the ray's reciprocal) the low plane is the near
+// one and for a < 0 the high one, which gives the same interval as taking
+// the per-axis min and max of the two plane values.
 __device__ inline float child_test_q(float ax, float bx, float ay, float by, float az, float bz, float t_best,
-                                     float qlx, float qhx, float qly, float qhy, float qlz, float qhz, uint32_t child) {
-    const float t0x = fmaf(qlx, ax, bx), t1x = fmaf(qhx, ax, bx);
-    const float t0y = fmaf(qly, ay, by), t1y = fmaf(qhy, ay, by);
-    const float t0z = fmaf(qlz, az, bz), t1z = fmaf(qhz, az, bz);
-    const float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
-    const float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), t_best));
+                                     float qnx, float qfx, float qny, float qfy, float qnz, float qfz,
+                                     uint32_t child) {
+    const float tnx = fmaf(qnx, ax, bx), tfx = fmaf(qfx, ax, bx);
+    const float tny = fmaf(qny, ay, by), tfy = fmaf(qfy, ay, by);
+    const float tnz = fmaf(qnz, az, bz), tfz = fmaf(qfz, az, bz);
+    const float tmin = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+    const float tmax = fminf(fminf(tfx, tfy), fminf(tfz, t_best));
     return (child != kNoChild && tmin <= tmax) ? tmin : __builtin_inff();
 }
 
@@ -412,12 +417,17 @@ __device__ inline void trace_persistent(const DevScene& sc, uint32_t n, uint32_t
                 const float ax = sx * L.inv4.x, ay = sy * L.inv4.y, az = sz * L.inv4.z;
                 const float bx = fmaf(ox, L.inv4.x, L.oi.x), by = fmaf(oy, L.inv4.y, L.oi.y),
                             bz = fmaf(oz, L.inv4.z, L.oi.z);
+                // plane words in near/far order for the ray's direction signs
+                const bool px = L.inv4.x >= 0.0f, py = L.inv4.y >= 0.0f, pz = L.inv4.z >= 0.0f;
+                const uint32_t nx = px ? qa.x : qa.y, fx = px ? qa.y : qa.x;
+                const uint32_t ny = py ? qa.z : qa.w, fy = py ? qa.w : qa.z;
+                const uint32_t nz = pz ? qb.x : qb.y, fz = pz ? qb.y : qb.x;
                 float k[4];
                 uint32_t e[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    k[c] = child_test_q(ax, bx, ay, by, az, bz, L.h.t, qbyte(qa.x, c), qbyte(qa.y, c), qbyte(qa.z, c),
-                                        qbyte(qa.w, c), qbyte(qb.x, c), qbyte(qb.y, c), e[c]);
+                    k[c] = child_test_q(ax, bx, ay, by, az, bz, L.h.t, qbyte(nx, c), qbyte(fx, c), qbyte(ny, c),
+                                        qbyte(fy, c), qbyte(nz, c), qbyte(fz, c), e[c]);
                     const uint32_t cw = (c < 2) ? qb.z : qb.w;
                     if ((cw >> (16 * (c & 1))) & 0xFFFFu) e[c] = kLeafBit | (L.cur << 2) | (uint32_t)c;
                 }
