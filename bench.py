@@ -358,6 +358,19 @@ def batch_bench(args):
            "scene_ms_min": round(float(ms.min()), 2), "scene_ms_median": round(float(np.median(ms)), 2),
            "scene_ms_max": round(float(ms.max()), 2), "tuning": tuning,
            "device_mb": round(st["device_bytes"] / 2 ** 20, 1)}
+    # the same render with the bounce loop captured once in a HIP graph and
+    # replayed (the tuner's other settings kept), whatever the tuner chose
+    img0 = b.image(0)
+    dev.set_tuning(**{**dev.get_tuning(), "use_graph": 1})
+    b.clear()
+    dev.sync()
+    t0 = time.perf_counter()
+    b.render()
+    dev.sync()
+    wall_g = time.perf_counter() - t0
+    res["graph_replay"] = {"wall_ms": round(wall_g * 1e3, 2), "mpaths_s": round(paths / wall_g / 1e6, 3),
+                           "scene0_image_max_rel_diff": float(np.max(np.abs(b.image(0) - img0)) /
+                                                              max(float(np.max(np.abs(img0))), 1e-30))}
     log(f"batch: {res}")
     b.close()
     dev.close()
@@ -508,17 +521,29 @@ def main():
     c1_film = dev.alloc(c1.camera.width * c1.camera.height * 12)
     p1 = c1.integrator.params()
     p1.flags = L.SPT_RENDER_FILM32
-    dev.render(c1.camera, p1, c1_film, None, None)
-    dev.sync()
-    t0 = time.perf_counter()
-    for _ in range(5):
-        dev.render(c1.camera, p1, c1_film, None, None)
-    dev.sync()
-    c1_s = (time.perf_counter() - t0) / 5
     c1_paths = c1.camera.width * c1.camera.height * c1.integrator.effective_spp()
+
+    def c1_time(reps=10):
+        dev.render(c1.camera, p1, c1_film, None, None)
+        dev.sync()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            dev.render(c1.camera, p1, c1_film, None, None)
+            dev.sync()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    c1_default = c1_time()
+    # a 1M-path image is latency-bound: the autotuner picks a smaller
+    # traversal grid and queue for it (the 1024-spp workload keeps its own)
+    c1_tuning = dev.autotune(c1.camera, c1.integrator, sample_paths=c1_paths)
+    c1_s = c1_time()
+    dev.set_tuning()  # back to the library defaults
     extra["box_icosphere_256x256"] = {"triangles": int(c1.n_tris), "spp": c1.integrator.effective_spp(),
                                       "max_depth": 5, "paths": c1_paths, "ms": round(c1_s * 1e3, 3),
-                                      "mpaths_s": round(c1_paths / c1_s / 1e6, 2)}
+                                      "mpaths_s": round(c1_paths / c1_s / 1e6, 2), "tuning": c1_tuning,
+                                      "default_tuning_mpaths_s": round(c1_paths / c1_default / 1e6, 2)}
     log(f"box + icosphere: {extra['box_icosphere_256x256']}")
     c1_film.free()
 
