@@ -311,6 +311,19 @@ def guide_experiment(args):
     st = dev.guide_stats()
     flops = st["trained_records"] * 76.8e3
     mfma = flops / max(st["train_kernel_ms"] * 1e-3, 1e-12) / 2.5e15
+    # inference: one more 144-spp guided pass (trained guide, no capture) with
+    # the inference launches timed by HIP events; 25,600 useful FLOP per query
+    # (32x64 + 64x64 + 64x64 + 64x40 multiply-adds)
+    p = integ.params()
+    p.spp, p.seed = 144, 0x1FE12
+    p.flags = L.SPT_RENDER_FILM32 | L.SPT_RENDER_GUIDED | L.SPT_RENDER_TIME_KERNELS
+    film.zero()
+    dev.render(cam, p, film, None, None)
+    dev.sync()
+    rs = dev.render_stats()
+    inf_tflops = rs["guide_queries"] * 25600 / max(rs["guide_ms"] * 1e-3, 1e-12) / 1e12
+    infer = {"queries": int(rs["guide_queries"]), "kernel_ms": round(rs["guide_ms"], 3),
+             "tflops": round(inf_tflops, 1), "mfma_util": round(inf_tflops / 2500.0, 4)}
     res = {"scene": f"slit box, {sc.n_tris} triangles, {W}x{H}", "reference_spp": osp,
            "plain_ms": round(t_plain * 1e3, 2), "plain_rmse": round(r_plain, 6),
            "plain_mpaths_s": round(npix * 1024 / t_plain / 1e6, 3),
@@ -320,7 +333,7 @@ def guide_experiment(args):
            "guided_train_ms": round(t_train * 1e3, 2), "train_steps": st["steps"],
            "warmup_passes": args.guide_warmup, "warmup_ms": round(t_warm * 1e3, 2),
            "train_kernel_ms": round(st["train_kernel_ms"], 3), "trained_records": st["trained_records"],
-           "train_mfma_util": round(mfma, 6), "final_loss": st["last_loss"],
+           "train_mfma_util": round(mfma, 6), "final_loss": st["last_loss"], "inference": infer,
            "curve": curve[:: max(1, len(curve) // 24)]}
     log(f"guide experiment: {res}")
     dev.close()

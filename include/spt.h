@@ -350,6 +350,7 @@ typedef struct spt_render_stats {
     uint64_t node_fetches;
     uint64_t tri_fetches;
     double guide_ms;       /* with SPT_RENDER_TIME_KERNELS and GUIDED: guide inference time */
+    uint64_t guide_queries; /* GUIDED with the material sort: hits the guide network evaluated */
 } spt_render_stats;
 spt_status spt_render_get_stats(spt_ctx* ctx, spt_render_stats* out);
 

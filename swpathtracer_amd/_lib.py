@@ -106,7 +106,7 @@ class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("rays", C.c_uint64), ("bounces", C.c_uint64),
                 ("stream_overflows", C.c_uint64), ("ms", C.c_double), ("extend_ms", C.c_double),
                 ("extend_launches", C.c_uint64), ("node_fetches", C.c_uint64), ("tri_fetches", C.c_uint64),
-                ("guide_ms", C.c_double)]
+                ("guide_ms", C.c_double), ("guide_queries", C.c_uint64)]
 
 
 class GuideConfig(C.Structure):

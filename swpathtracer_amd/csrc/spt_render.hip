@@ -223,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void k_generate(RenderArgs a, PathBuf A, Wo
 // buffers; [2] replay overflows; [8] ray-fetch counter of the extend kernel;
 // u64 [10] rays, u64 [12] bounces; [16] guide records, [17] candidates;
 // u64 [20] next path to generate; [24] first slot and [25] number of the
-// planned new paths, u64 [26] their first path index; [28] plans made.
+// planned new paths, u64 [26] their first path index; [28] plans made;
+// u64 [30] hits evaluated by the guide network.
 constexpr int kGen = 20, kPlanSlot = 24, kPlanN = 25, kPlanPath = 26, kPlanSeq = 28;
 
 // Between bounces (one thread): the consumed buffer's count and the
@@ -415,6 +416,18 @@ __global__ __launch_bounds__(kBlock) void k_scatter(RenderArgs a, Work w, const 
         }
         __syncthreads();
     }
+}
+
+// Guided calls: the hits the guide network evaluates this bounce (those on
+// guidable materials), read off the sort's exclusive offsets ([key][chunk]:
+// the count of key m is the first offset of m + 1 minus that of m), added to
+// the u64 at work word 30. One thread.
+__global__ void k_count_guided(RenderArgs a, Work w, uint32_t nb) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long q = 0;
+    for (uint32_t m = 0; m + 1 < a.keys; ++m)  // the last key is the misses
+        if (guide_mode(a, m) != 0u) q += w.bhist[(m + 1) * nb] - w.bhist[m * nb];
+    if (q) atomicAdd(reinterpret_cast<unsigned long long*>(w.cnt + 30), q);
 }
 
 // Guide inference for the live hits of buffer A whose material is guidable,
@@ -1000,6 +1013,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
             hipLaunchKernelGGL(k_hist, dim3(sort_blocks), dim3(kBlock), 0, st, a, w, cntA);
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, a, w, sort_blocks);
             hipLaunchKernelGGL(k_scatter, dim3(sort_blocks), dim3(kBlock), 0, st, a, w, cntA);
+            if (a.guided) hipLaunchKernelGGL(k_count_guided, dim3(1), dim3(64), 0, st, a, w, sort_blocks);
         }
         if (a.guided) {
             if (timed) {
@@ -1115,7 +1129,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
         drop_graph();
         return s;
     }
-    uint32_t cnt[20];  // [16]: guide records written, [17]: candidate vertices
+    uint32_t cnt[32];  // the work words above
     s = read_small(ctx, cnt, w.cnt, sizeof(cnt), "render");
     drop_graph();
     if (s != SPT_OK) return s;
@@ -1137,6 +1151,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
     stats.rays = (uint64_t)cnt[10] | ((uint64_t)cnt[11] << 32);
     if (a.train && total) ctx->guide.rays_per_path = std::max(1.0, (double)stats.rays / (double)total);
     stats.bounces = (uint64_t)cnt[12] | ((uint64_t)cnt[13] << 32);
+    stats.guide_queries = (uint64_t)cnt[30] | ((uint64_t)cnt[31] << 32);
     if (timed) {
         for (uint64_t i = 0; i < stats.extend_launches; ++i) {
             float ms = 0.0f;
