@@ -684,7 +684,7 @@ struct GuideState {
     double rays_per_path = 4.0;    // extension rays per path of the last training render
     uint32_t batch = 262144;
     double train_ms = 0.0;
-    float* d_gout = nullptr;  // per-path raw outputs during a guided render
+    unsigned short* d_gout = nullptr;  // per-path raw outputs (half floats) during a guided render
     size_t gout_cap = 0;
     uint32_t rec_stride = 1;      // 1 in rec_stride candidate vertices is recorded
     double cand_per_path = 0.0;   // candidate vertices per path of the last training render

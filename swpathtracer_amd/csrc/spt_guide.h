@@ -10,6 +10,8 @@
 // of a sampling density on Monte Carlo targets after Mueller et al. (2019).
 #pragma once
 
+#include <hip/hip_fp16.h>
+
 #include "spt_device.h"
 
 namespace spt {
@@ -221,6 +223,25 @@ __device__ inline void store_outputs(float* dst, const f32x4 y[3]) {
     if (g < 2) *reinterpret_cast<float4*>(dst + 32 + 4 * g) = make_float4(y[2][0], y[2][1], y[2][2], y[2][3]);
 }
 
+// The same outputs as IEEE half floats (round to nearest), 80 B per sample:
+// the per-path copy a guided render keeps between inference and shading.
+__device__ inline uint32_t pack_h2(float a, float b) {
+    return (uint32_t)__half_as_ushort(__float2half_rn(a)) | ((uint32_t)__half_as_ushort(__float2half_rn(b)) << 16);
+}
+__device__ inline void store_outputs_h(unsigned short* dst, const f32x4 y[3]) {
+    const int g = __lane_id() >> 4;
+    *reinterpret_cast<uint2*>(dst + 4 * g) = make_uint2(pack_h2(y[0][0], y[0][1]), pack_h2(y[0][2], y[0][3]));
+    *reinterpret_cast<uint2*>(dst + 16 + 4 * g) = make_uint2(pack_h2(y[1][0], y[1][1]), pack_h2(y[1][2], y[1][3]));
+    if (g < 2)
+        *reinterpret_cast<uint2*>(dst + 32 + 4 * g) = make_uint2(pack_h2(y[2][0], y[2][1]), pack_h2(y[2][2], y[2][3]));
+}
+
+// Output k of a half-float row as float.
+struct HalfRow {
+    const unsigned short* p;
+    __device__ float operator[](int k) const { return __half2float(__ushort_as_half(p[k])); }
+};
+
 // Forward of the 64 samples staged by this wave: for sample group q (16
 // samples 16 q .. 16 q + 15) calls out(q, y) with lane (r, g) holding outputs
 // 16 t + 4 g + i of sample 16 q + r.
@@ -244,7 +265,9 @@ struct Lobe {
     bool inside;  // log kappa inside its clamp range (gradient flows)
 };
 
-__device__ inline void mixture_params(const float* y, Lobe L[kLobes]) {
+// y: the 40 raw outputs (a float pointer, or a HalfRow).
+template <typename Row>
+__device__ inline void mixture_params(const Row& y, Lobe L[kLobes]) {
     float amax = -3.0e38f;
     for (int i = 0; i < kLobes; ++i) amax = fmaxf(amax, y[5 * i + 4]);
     float sum = 0.0f;

@@ -66,7 +66,7 @@ struct RenderArgs {
     uint32_t gflags;          // SPT_GUIDE_*
     const uint8_t* guidable;  // per material: kGuideSingle / kGuideCoated / 0
     float bsdf_frac;
-    float* gout;              // per path slot: 40 raw network outputs
+    unsigned short* gout;     // per path slot: 40 raw network outputs (IEEE half)
     float* records;           // training records (guide::kRecord floats each)
     uint32_t* rec_count;  // [0]: records written, [1]: candidate vertices
     uint32_t rec_cap;
@@ -432,7 +432,7 @@ __global__ void k_count_guided(RenderArgs a, Work w, uint32_t nb) {
 
 // Guide inference for the live hits of buffer A whose material is guidable,
 // in shading order: each wave encodes 64 hits (one per lane) and runs the
-// register-resident MLP; raw outputs go to gout[slot].
+// register-resident MLP; raw outputs go to gout[slot] as half floats.
 __global__ __launch_bounds__(kBlock) void k_guide_paths(RenderArgs a, PathBuf A, Work w,
                                                         const uint32_t* __restrict__ cntA,
                                                         const float* __restrict__ theta,
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_guide_paths(RenderArgs a, PathBuf A,
             mlp_stage_fwd(sw, st, [&](int g16, const f32x4 y[3]) {
                 const int src = 16 * g16 + r;
                 const uint32_t s = __shfl(slot, src);
-                if ((any >> src) & 1ull) store_outputs(a.gout + (size_t)s * kOut, y);
+                if ((any >> src) & 1ull) store_outputs_h(a.gout + (size_t)s * kOut, y);
             });
         __syncthreads();
     }
@@ -683,7 +683,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             // lobe in exit space
             using namespace guide;
             Lobe lobes[kLobes];
-            mixture_params(a.gout + (size_t)i * kOut, lobes);
+            mixture_params(HalfRow{a.gout + (size_t)i * kOut}, lobes);
             V3 mb;
             float kb;
             bsdf_lobe(m.layers[1], wo, m.layers[0].ior, mb, kb);
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             using namespace guide;
             const spt_layer& l = m.layers[0];
             Lobe lobes[kLobes];
-            mixture_params(a.gout + (size_t)i * kOut, lobes);
+            mixture_params(HalfRow{a.gout + (size_t)i * kOut}, lobes);
             if (prod) {
                 V3 mb;
                 float kb;
@@ -928,7 +928,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
     a.rec_count = w.cnt + 16;
     a.refill = std::min<uint32_t>(64, std::max<uint32_t>(1, tn.refill_lanes));
     if (a.guided) {
-        const size_t need = (size_t)cap * guide::kOut * sizeof(float);
+        const size_t need = (size_t)cap * guide::kOut * sizeof(unsigned short);
         if (need > ctx->guide.gout_cap) {
             (void)hipStreamSynchronize(ctx->stream);
             if (ctx->guide.d_gout) (void)hipFree(ctx->guide.d_gout);
