@@ -319,3 +319,22 @@ def test_in_call_training_trains_during_the_render(device):
     assert np.isfinite(st["last_loss"])
     assert np.all(np.isfinite(img))
     assert device.guide_train_collected() == 0.0
+
+
+@pytest.mark.gpu
+def test_guide_queries_are_counted(device):
+    """spt_render_stats.guide_queries: hits the guide network evaluated in a
+    guided call (counted from the material sort's offsets): positive, at most
+    the extension rays, the same for a repeated call, zero without GUIDED."""
+    sc = scenes.slit_box(rings=32, segments=64, width=48, height=36, spp=4)
+    device.set_scene(api.Bvh(sc.verts), sc.normals, sc.mat_ids)
+    device.set_materials(sc.materials)
+    device.guide_init(seed=1, batch=65536)
+    device.render_image(sc.camera, sc.integrator, extra_flags=L.SPT_RENDER_GUIDED)
+    st1 = device.render_stats()
+    device.render_image(sc.camera, sc.integrator, extra_flags=L.SPT_RENDER_GUIDED)
+    st2 = device.render_stats()
+    assert 0 < st1["guide_queries"] <= st1["rays"]
+    assert st1["guide_queries"] == st2["guide_queries"]
+    device.render_image(sc.camera, sc.integrator)
+    assert device.render_stats()["guide_queries"] == 0
