@@ -380,6 +380,10 @@ typedef struct spt_tuning {
     uint32_t refill_lanes;         /* a traversal wave fetches new rays once this many of
                                       its 64 lanes are idle (default 48 in path tracing,
                                       16 in spt_trace) */
+    uint32_t wavefronts;           /* unguided renders: independent wavefronts run at once on
+                                      their own streams, each with a share of the paths and of
+                                      the queue capacity, so one's traversal overlaps another's
+                                      shading (1..4, default 3) */
 } spt_tuning;
 spt_status spt_set_tuning(spt_ctx* ctx, const spt_tuning* t);
 spt_status spt_get_tuning(spt_ctx* ctx, spt_tuning* t);

@@ -125,7 +125,8 @@ class GuideStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("path_capacity", C.c_uint32), ("extend_blocks_per_cu", C.c_uint32),
                 ("shade_blocks_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("tile_pixels", C.c_uint32),
-                ("tile_samples", C.c_uint32), ("check_every", C.c_uint32), ("refill_lanes", C.c_uint32)]
+                ("tile_samples", C.c_uint32), ("check_every", C.c_uint32), ("refill_lanes", C.c_uint32),
+                ("wavefronts", C.c_uint32)]
 
 
 class BatchScene(C.Structure):

@@ -416,6 +416,8 @@ spt_status spt_create(int device, spt_ctx** out) {
 
 void spt_destroy(spt_ctx* ctx) {
     if (!ctx) return;
+    for (spt_ctx* f : ctx->fronts) spt_destroy(f);
+    ctx->fronts.clear();
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     free_scene(ctx);

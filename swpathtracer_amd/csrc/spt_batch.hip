@@ -81,6 +81,7 @@ spt_status tune(spt_ctx* ctx, bool batch, F run, spt_tuning* out) {
         {&spt_tuning::use_graph, {2u, 1u}},
         {&spt_tuning::check_every, {2u, 4u, 8u}},
         {&spt_tuning::refill_lanes, {16u, 32u, 48u, 56u}},
+        {&spt_tuning::wavefronts, {3u, 2u, 1u}},
     };
     if (batch) {
         dims.push_back({&spt_tuning::tile_pixels, {1024u, 4096u, 16384u, 65536u}});
@@ -119,7 +120,7 @@ extern "C" {
 spt_status spt_set_tuning(spt_ctx* ctx, const spt_tuning* t) {
     if (!ctx || !t) return SPT_ERR_INVALID;
     if (t->path_capacity > (1u << 27) || t->use_graph > 2 || t->shade_blocks_per_cu > 1024 ||
-        t->extend_blocks_per_cu > 64 || t->check_every > 64 || t->refill_lanes > 64)
+        t->extend_blocks_per_cu > 64 || t->check_every > 64 || t->refill_lanes > 64 || t->wavefronts > 4)
         return set_error(ctx, SPT_ERR_INVALID, "tuning value out of range");
     ctx->tuning = *t;
     return SPT_OK;

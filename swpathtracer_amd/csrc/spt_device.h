@@ -777,6 +777,9 @@ struct spt_ctx {
     // mapped host memory the bounce loop reports its progress to (see k_plan)
     volatile uint32_t* h_status = nullptr;
     uint32_t* d_status = nullptr;
+    // contexts of the extra concurrent wavefronts of a render (their own
+    // stream, queue, status, flags and spill buffer), created on first use
+    std::vector<spt_ctx*> fronts;
 
     spt::DevScene scene() const {
         spt::DevScene s;

@@ -10,6 +10,7 @@
 // throughput: L = sum_d (prod_{j<d} f_j cos_j / pdf_j) * Le_d.
 #include <algorithm>
 #include <chrono>
+#include <thread>
 
 #include "spt_bsdf.h"
 #include "spt_guide.h"
@@ -61,6 +62,7 @@ struct RenderArgs {
     uint32_t per_path;  // some job writes per-path radiance: paths carry their output index
     uint32_t refill;  // idle lanes that trigger a ray refill in the traversal
     uint32_t keys;  // sort keys (n_mats + 1), 0 = no sort
+    uint64_t gen_base;  // number of this wavefront's first path in the call's numbering
     // learned guide
     uint32_t guided, train;
     uint32_t gflags;          // SPT_GUIDE_*
@@ -160,6 +162,7 @@ __device__ inline Draws make_draws(const Job& jb, uint64_t key, uint32_t k) {
 // With max depth 0 (single-scene calls only) the path finishes here.
 template <bool MULTI>
 __device__ inline void gen_path(const RenderArgs& a, PathBuf& A, const Work& w, uint32_t s, uint64_t g) {
+    g += a.gen_base;  // a wavefront numbers its own paths from 0
     uint32_t jid = 0;
     uint64_t pl, local;
     uint32_t smp;
@@ -211,8 +214,8 @@ __device__ inline void gen_path(const RenderArgs& a, PathBuf& A, const Work& w, 
     if (a.train) A.grec[s] = make_uint4(kNoRec, kNoRec, kNoRec, 0u);
 }
 
-// Paths [base, base + n) into slots [0, n) of buffer A; a call with max depth
-// 0 finishes them all here.
+// Paths [base, base + n) (of this wavefront) into slots [0, n) of buffer A;
+// a call with max depth 0 finishes them all here.
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_generate(RenderArgs a, PathBuf A, Work w, uint64_t base, uint32_t n) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
@@ -810,6 +813,7 @@ spt_tuning resolve_tuning(const spt_tuning& t) {
     if (r.tile_pixels == 0) r.tile_pixels = 4096;
     if (r.tile_samples == 0) r.tile_samples = 16;
     if (r.check_every == 0) r.check_every = 4;
+    if (r.wavefronts == 0) r.wavefronts = 3;
     if (r.refill_lanes == 0) r.refill_lanes = 48;
     return r;
 }
@@ -946,7 +950,10 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
                                                      kBlock, 0) != hipSuccess ||
         per_cu <= 0)
         per_cu = 4;
-    if (tn.extend_blocks_per_cu) per_cu = std::min<int>(per_cu, (int)tn.extend_blocks_per_cu);
+    if (tn.extend_blocks_per_cu)
+        per_cu = std::min<int>(per_cu, (int)tn.extend_blocks_per_cu);
+    else if (tn.wavefronts > 1)  // concurrent wavefronts share the CUs
+        per_cu = std::max<int>(1, (per_cu + (int)tn.wavefronts - 1) / (int)tn.wavefronts + 1);
     const uint32_t ext_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)(per_cu * ctx->num_cus));
     const uint32_t shade_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * tn.shade_blocks_per_cu);
     const uint32_t refill_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * 8);
@@ -1176,6 +1183,83 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
     return SPT_OK;
 }
 
+// Runs `total` paths as tn.wavefronts independent wavefronts at once: each
+// on its own stream, with its own queue (the capacity split between them),
+// status word, flags and stack spill buffer, taking a contiguous share of
+// the call's path numbering. One wavefront's traversal (bound by VALU issue)
+// then overlaps another's shading (bound by memory latency). Per-path results
+// do not depend on the wavefront a path runs in. Guided calls (one record
+// ring, one output buffer) and calls of fewer than 2^21 paths per wavefront
+// run as one.
+template <bool MULTI>
+spt_status run_fronts(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuning& tn, spt_render_stats& stats) {
+    spt_tuning t1 = tn;
+    uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(4, tn.wavefronts));
+    if (a.guided) k = 1;
+    while (k > 1 && total / k < (1ull << 21)) --k;
+    if (k == 1) {
+        t1.wavefronts = 1;
+        return run_paths<MULTI>(ctx, a, total, t1, stats);
+    }
+    while (ctx->fronts.size() < k - 1) {
+        spt_ctx* f = nullptr;
+        const spt_status cs = spt_create(ctx->device, &f);
+        if (cs != SPT_OK) return set_error(ctx, cs, "context of a concurrent wavefront");
+        ctx->fronts.push_back(f);
+    }
+    t1.wavefronts = k;
+    t1.path_capacity = std::max<uint32_t>(1u << 20, tn.path_capacity / k);
+    std::vector<RenderArgs> args(k, a);
+    std::vector<spt_render_stats> st(k, stats);
+    std::vector<spt_status> res(k, SPT_OK);
+    std::vector<uint64_t> n(k);
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint64_t lo = total * i / k, hi = total * (i + 1) / k;
+        args[i].gen_base = lo;
+        n[i] = hi - lo;
+    }
+    // the stream of the calling context orders this call after earlier work
+    // on it; the other wavefronts start once that work is done
+    spt_status s = stream_wait(ctx, "render start");
+    if (s != SPT_OK) return s;
+    auto run = [&](uint32_t i) {
+        if (hipSetDevice(ctx->device) != hipSuccess) {
+            res[i] = SPT_ERR_DEVICE;
+            return;
+        }
+        res[i] = run_paths<MULTI>(i ? ctx->fronts[i - 1] : ctx, args[i], n[i], t1, st[i]);
+    };
+    std::vector<std::thread> th;
+    std::vector<uint32_t> inline_fronts;
+    for (uint32_t i = 1; i < k; ++i) {
+        try {
+            th.emplace_back(run, i);
+        } catch (...) {  // no thread: that wavefront runs after the others
+            inline_fronts.push_back(i);
+        }
+    }
+    run(0);
+    for (std::thread& x : th) x.join();
+    for (uint32_t i : inline_fronts) run(i);
+    for (uint32_t i = 0; i < k; ++i) {
+        if (res[i] != SPT_OK) {
+            if (i > 0) ctx->err = ctx->fronts[i - 1]->err;
+            return res[i];
+        }
+    }
+    stats = st[0];
+    for (uint32_t i = 1; i < k; ++i) {
+        stats.rays += st[i].rays;
+        stats.bounces = std::max(stats.bounces, st[i].bounces);
+        stats.stream_overflows += st[i].stream_overflows;
+        stats.extend_ms += st[i].extend_ms;
+        stats.extend_launches += st[i].extend_launches;
+        stats.node_fetches += st[i].node_fetches;
+        stats.tri_fetches += st[i].tri_fetches;
+    }
+    return SPT_OK;
+}
+
 }  // namespace
 
 spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_params* prm, float* d_rgb32,
@@ -1228,7 +1312,7 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
     a.keys = (!(prm->flags & SPT_RENDER_NO_SORT) && keys <= kMaxSortKeys) ? keys : 0u;
     spt_render_stats stats{};
     stats.paths = total;
-    s = run_paths<false>(ctx, a, total, resolve_tuning(ctx->tuning), stats);
+    s = run_fronts<false>(ctx, a, total, resolve_tuning(ctx->tuning), stats);
     stats.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->stats = stats;
     return s;
@@ -1332,7 +1416,7 @@ spt_status launch_batch(spt_ctx* ctx, BatchData& b, uint32_t sample_div) {
     a.keys = keys <= kMaxSortKeys ? keys : 0u;
     spt_render_stats stats{};
     stats.paths = total;
-    s = run_paths<true>(ctx, a, total, tn, stats);
+    s = run_fronts<true>(ctx, a, total, tn, stats);
     if (s != SPT_OK) return s;
     std::vector<uint64_t> clk(nj + 1, 0);
     std::vector<uint32_t> done(nj, 0);
