@@ -314,6 +314,10 @@ def guide_experiment(args):
     # inference: one more 144-spp guided pass (trained guide, no capture) with
     # the inference launches timed by HIP events; 25,600 useful FLOP per query
     # (32x64 + 64x64 + 64x64 + 64x40 multiply-adds)
+    # (one wavefront, so that no other wavefront's kernels share the timed
+    # launches)
+    tn0 = dev.get_tuning()
+    dev.set_tuning(**{**tn0, "wavefronts": 1})
     p = integ.params()
     p.spp, p.seed = 144, 0x1FE12
     p.flags = L.SPT_RENDER_FILM32 | L.SPT_RENDER_GUIDED | L.SPT_RENDER_TIME_KERNELS
@@ -321,6 +325,7 @@ def guide_experiment(args):
     dev.render(cam, p, film, None, None)
     dev.sync()
     rs = dev.render_stats()
+    dev.set_tuning(**tn0)
     inf_tflops = rs["guide_queries"] * 25600 / max(rs["guide_ms"] * 1e-3, 1e-12) / 1e12
     infer = {"queries": int(rs["guide_queries"]), "kernel_ms": round(rs["guide_ms"], 3),
              "tflops": round(inf_tflops, 1), "mfma_util": round(inf_tflops / 2500.0, 4)}

@@ -10,6 +10,7 @@
 // throughput: L = sum_d (prod_{j<d} f_j cos_j / pdf_j) * Le_d.
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <thread>
 
 #include "spt_bsdf.h"
@@ -71,6 +72,10 @@ struct RenderArgs {
     unsigned short* gout;     // per path slot: 40 raw network outputs (IEEE half)
     float* records;           // training records (guide::kRecord floats each)
     uint32_t* rec_count;  // [0]: records written, [1]: candidate vertices
+    uint32_t* rec_shared; // concurrent wavefronts: one rec_count for all (else nullptr: the work words)
+    const float* theta;          // guide parameters (FP32 master copy)
+    const unsigned short* wbf;   // guide weights in bf16
+    GuideBox gbox;               // encoding box of the guide
     uint32_t rec_cap;
     uint32_t rec_stride;  // record 1 in rec_stride candidates (hash of key and depth)
     uint32_t rec_ring;    // in-call training: record slots wrap around rec_cap
@@ -879,6 +884,18 @@ spt_status make_job(spt_ctx* ctx, const spt_camera* cam, const spt_render_params
 
 namespace {
 
+// The dynamic LDS size of the guide inference kernel, set once per process
+// (thread-safe: concurrent wavefronts start their calls together).
+spt_status allow_guide_paths_lds(spt_ctx* ctx) {
+    static std::once_flag once;
+    static hipError_t e = hipSuccess;
+    std::call_once(once, []() {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_guide_paths),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)guide_paths_lds_bytes());
+    });
+    return check_hip(ctx, e, "guide LDS size");
+}
+
 // Runs `total` paths through the wavefront: the queue is filled to its
 // capacity, then every bounce refills it from the paths not yet generated,
 // until every path is generated and none is live.
@@ -929,7 +946,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
     w.cnt = (uint32_t*)take(256);
     if ((size_t)(base - reinterpret_cast<char*>(ctx->d_paths)) > ctx->paths_bytes)
         return set_error(ctx, SPT_ERR_NOMEM, "path state layout exceeds its allocation");
-    a.rec_count = w.cnt + 16;
+    a.rec_count = a.rec_shared ? a.rec_shared : w.cnt + 16;
     a.refill = std::min<uint32_t>(64, std::max<uint32_t>(1, tn.refill_lanes));
     if (a.guided) {
         const size_t need = (size_t)cap * guide::kOut * sizeof(unsigned short);
@@ -968,23 +985,15 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
     const bool fetch_stats = (a.flags & SPT_RENDER_FETCH_STATS) != 0;
     s = check_hip(ctx, hipMemsetAsync(w.cnt, 0, 256, st), "counters");
     if (s != SPT_OK) return s;
-    if (a.train) {  // records continue after those collected since the last training step
+    if (a.train && !a.rec_shared) {  // records continue after those collected since the last training step
         s = check_hip(ctx,
                       hipMemsetD32Async((hipDeviceptr_t)(w.cnt + 16), a.rec_ring ? 0 : (int)ctx->guide.collected, 1, st),
                       "record counter");
         if (s != SPT_OK) return s;
     }
     if (a.guided) {
-        static bool lds_ok = false;
-        if (!lds_ok) {
-            s = check_hip(ctx,
-                          hipFuncSetAttribute(reinterpret_cast<const void*>(&k_guide_paths),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)guide_paths_lds_bytes()),
-                          "guide LDS size");
-            if (s != SPT_OK) return s;
-            lds_ok = true;
-        }
+        s = allow_guide_paths_lds(ctx);
+        if (s != SPT_OK) return s;
     }
     if (fetch_stats) {
         s = check_hip(ctx, hipMemsetAsync(ctx->d_flags + 4, 0, 16, st), "fetch counters");
@@ -1032,7 +1041,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
                 }
             }
             hipLaunchKernelGGL(k_guide_paths, dim3((uint32_t)ctx->num_cus * 2), dim3(kBlock), guide_paths_lds_bytes(),
-                               st, a, A, w, cntA, ctx->guide.d_theta, ctx->guide.d_wbf, guide_box(ctx));
+                               st, a, A, w, cntA, a.theta, a.wbf, a.gbox);
             if (timed && !gev.empty()) (void)hipEventRecord(gev.back(), st);
         }
         if (a.guided)
@@ -1141,7 +1150,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
     drop_graph();
     if (s != SPT_OK) return s;
     stats.stream_overflows = cnt[2];
-    if (a.train) {
+    if (a.train && !a.rec_shared) {
         GuideState& gs = ctx->guide;
         // a ring of records was trained inside the call; otherwise the
         // records wait for spt_guide_train_collected
@@ -1156,7 +1165,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
     }
     for (hipEvent_t e : gev) (void)hipEventDestroy(e);
     stats.rays = (uint64_t)cnt[10] | ((uint64_t)cnt[11] << 32);
-    if (a.train && total) ctx->guide.rays_per_path = std::max(1.0, (double)stats.rays / (double)total);
+    if (a.train && !a.rec_shared && total) ctx->guide.rays_per_path = std::max(1.0, (double)stats.rays / (double)total);
     stats.bounces = (uint64_t)cnt[12] | ((uint64_t)cnt[13] << 32);
     stats.guide_queries = (uint64_t)cnt[30] | ((uint64_t)cnt[31] << 32);
     if (timed) {
@@ -1195,7 +1204,7 @@ template <bool MULTI>
 spt_status run_fronts(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuning& tn, spt_render_stats& stats) {
     spt_tuning t1 = tn;
     uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(4, tn.wavefronts));
-    if (a.guided) k = 1;
+    if (a.rec_ring) k = 1;  // in-call training trains from one wavefront's ring
     while (k > 1 && total / k < (1ull << 21)) --k;
     if (k == 1) {
         t1.wavefronts = 1;
@@ -1217,6 +1226,20 @@ spt_status run_fronts(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tun
         const uint64_t lo = total * i / k, hi = total * (i + 1) / k;
         args[i].gen_base = lo;
         n[i] = hi - lo;
+    }
+    // training records of all wavefronts go to one buffer through one
+    // counter pair (records written, candidate vertices) in the calling
+    // context's flag words 12-13, starting after the records collected so far
+    if (a.train) {
+        const uint32_t init[2] = {(uint32_t)ctx->guide.collected, 0u};
+        spt_status cs = check_hip(ctx, hipMemcpyAsync(ctx->d_flags + 12, init, sizeof(init), hipMemcpyHostToDevice,
+                                                      ctx->stream), "record counter");
+        if (cs != SPT_OK) return cs;
+        for (RenderArgs& x : args) x.rec_shared = ctx->d_flags + 12;
+    }
+    if (a.guided) {
+        spt_status ls = allow_guide_paths_lds(ctx);
+        if (ls != SPT_OK) return ls;
     }
     // the stream of the calling context orders this call after earlier work
     // on it; the other wavefronts start once that work is done
@@ -1256,6 +1279,18 @@ spt_status run_fronts(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tun
         stats.extend_launches += st[i].extend_launches;
         stats.node_fetches += st[i].node_fetches;
         stats.tri_fetches += st[i].tri_fetches;
+        stats.guide_ms += st[i].guide_ms;
+        stats.guide_queries += st[i].guide_queries;
+    }
+    if (a.train) {
+        uint32_t rc[2];
+        s = read_small(ctx, rc, ctx->d_flags + 12, sizeof(rc), "record counter");
+        if (s != SPT_OK) return s;
+        GuideState& gs = ctx->guide;
+        gs.collected = std::min<uint64_t>(rc[0], a.rec_cap);
+        gs.candidates = rc[1];
+        gs.cand_per_path = total ? (double)rc[1] / (double)total : 0.0;
+        if (total) gs.rays_per_path = std::max(1.0, (double)stats.rays / (double)total);
     }
     return SPT_OK;
 }
@@ -1283,6 +1318,9 @@ spt_status launch_render(spt_ctx* ctx, const spt_camera* cam, const spt_render_p
         return set_error(ctx, SPT_ERR_INVALID, "GUIDE_TRAIN needs GUIDED");
     if (a.guided && !ctx->guide.ready) return set_error(ctx, SPT_ERR_STATE, "guided rendering needs spt_guide_init");
     a.guidable = ctx->d_guidable;
+    a.theta = ctx->guide.d_theta;
+    a.wbf = ctx->guide.d_wbf;
+    a.gbox = guide_box(ctx);
     a.bsdf_frac = ctx->guide.bsdf_fraction;
     a.gflags = ctx->guide.flags;
     a.records = ctx->guide.d_records;
