@@ -517,6 +517,22 @@ def main():
     # per extension ray: queue index 4 + origin 16 + direction 16 in, hit 16 + sort key 4 out
     bytes_per_ray = 56 + 32 * per_ray_nodes + 48 * per_ray_tris  # tree data counted in 32-B units
     achieved = bytes_per_ray * rays / (ext_ms * 1e-3) / 1e9
+    # The renderer runs concurrent wavefronts, so an extension launch shares
+    # the GPU with another wavefront's shading and lasts longer than it would
+    # alone. Beside the per-launch figure: the extension rays' algorithmic
+    # bytes over the whole timed wall time, and one untimed step on a single
+    # wavefront (the kernel with the GPU to itself).
+    aggregate = bytes_per_ray * rays / wall / 1e9
+    tn_default = dev.get_tuning()
+    dev.set_tuning(**{**tn_default, "wavefronts": 1})
+    step(0, L.SPT_RENDER_FILM32 | L.SPT_RENDER_TIME_KERNELS)
+    dev.sync()
+    si = dev.render_stats()
+    dev.set_tuning()  # back to the library defaults
+    single = {"avg_launch_ms": round(si["extend_ms"] / max(si["extend_launches"], 1), 4),
+              "launches": int(si["extend_launches"]),
+              "achieved": round(bytes_per_ray * si["rays"] / (si["extend_ms"] * 1e-3) / 1e9, 1)}
+    single["frac"] = round(single["achieved"] / HBM_PEAK_GBS, 4)
 
     # HBM bytes per extension launch of this workload, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated)
@@ -525,7 +541,8 @@ def main():
     if os.path.exists(tf) and args.rings == 512 and args.segments == 1024:
         with open(tf) as fh:
             tj = json.load(fh)
-        if tj.get("spp_per_step") == sps:  # measured on this workload's launches
+        # measured on this workload's launches, with as many wavefronts
+        if tj.get("spp_per_step") == sps and tj.get("wavefronts", 1) == tn_default["wavefronts"]:
             traffic = tj["hbm_bytes_per_launch"]
 
     extra = {}
@@ -620,7 +637,10 @@ def main():
                      "kernel": "k_extend (closest-hit traversal of extension rays)",
                      "bytes_per_ray": round(bytes_per_ray, 1), "tree_units_32B_per_ray": round(per_ray_nodes, 2),
                      "tri_fetches_per_ray": round(per_ray_tris, 2),
-                     "avg_launch_ms": round(ext_ms / max(ext_n, 1), 4), "launches": ext_n},
+                     "avg_launch_ms": round(ext_ms / max(ext_n, 1), 4), "launches": ext_n,
+                     "wavefronts": tn_default["wavefronts"],
+                     "aggregate_achieved": round(aggregate, 1), "aggregate_frac": round(aggregate / HBM_PEAK_GBS, 4),
+                     "single_wavefront": single},
         "cpu_baseline": cpu,
         "image_mean": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)],
     }
