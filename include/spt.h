@@ -370,16 +370,18 @@ typedef struct spt_tuning {
                                       about 250 B of device memory per slot, 410 B when guided;
                                       a render allocates min(paths of the call, capacity), halved
                                       while the device has no room, down to 2^20) */
-    uint32_t extend_blocks_per_cu; /* persistent traversal grid (default: occupancy limit) */
-    uint32_t shade_blocks_per_cu;  /* shading grid, at most 1024 (default 128) */
+    uint32_t extend_blocks_per_cu; /* persistent traversal grid (default: the occupancy limit on
+                                      one wavefront, at most 4 each for two, 2 for more) */
+    uint32_t shade_blocks_per_cu;  /* shading grid, at most 1024 (default 128 on one wavefront,
+                                      64 with several) */
     uint32_t use_graph;            /* 1: bounce loop captured once in a HIP graph and replayed;
                                       2: plain launches (default) */
     uint32_t tile_pixels;          /* multi-scene calls: pixels per generation tile (default 4096) */
     uint32_t tile_samples;         /* multi-scene calls: samples per tile (default 16) */
     uint32_t check_every;          /* bounces the host may run ahead of the device (default 4) */
     uint32_t refill_lanes;         /* a traversal wave fetches new rays once this many of
-                                      its 64 lanes are idle (default 48 in path tracing,
-                                      16 in spt_trace) */
+                                      its 64 lanes are idle (default in path tracing 48 on one
+                                      wavefront, 20 with several; 16 in spt_trace) */
     uint32_t wavefronts;           /* unguided renders: independent wavefronts run at once on
                                       their own streams, each with a share of the paths and of
                                       the queue capacity, so one's traversal overlaps another's
