@@ -156,45 +156,78 @@ __device__ inline f32x4 mfma_k16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Inference weights in MFMA A-operand order. For a layer of IN inputs, output
+// tile t and k-tile pair p (tiles 2p and 2p + 1) one 1-KB block holds, for
+// lane r + 16 g, the 8 bf16 W[16 t + r][32 p + 4 g + 0..3] and
+// W[16 t + r][32 p + 16 + 4 g + 0..3]: each MFMA's weight operand is one
+// 16-B LDS read per lane at consecutive addresses (the row-major layout
+// puts the 16 rows of a read 64 or 128 B apart, on the same banks).
+struct FwdWeights {
+    uint4 w1[4 * (kEnc / 32) * 64];
+    uint4 w2[4 * (kHid / 32) * 64];
+    uint4 w3[4 * (kHid / 32) * 64];
+    uint4 w4[(kOutP / 16) * (kHid / 32) * 64];
+    float b1[kHid], b2[kHid], b3[kHid], b4[kOutP];
+};
+
+template <int IN>
+__device__ inline void swizzle_layer(uint4* dst, const unsigned short* w, int tiles) {
+    for (int i = threadIdx.x; i < tiles * (IN / 32) * 64; i += blockDim.x) {
+        const int l = i & 63, p = (i >> 6) % (IN / 32), t = (i >> 6) / (IN / 32);
+        const unsigned short* row = w + (16 * t + (l & 15)) * IN + 32 * p + 4 * (l >> 4);
+        dst[i] = make_uint4((uint32_t)row[0] | ((uint32_t)row[1] << 16), (uint32_t)row[2] | ((uint32_t)row[3] << 16),
+                            (uint32_t)row[16] | ((uint32_t)row[17] << 16),
+                            (uint32_t)row[18] | ((uint32_t)row[19] << 16));
+    }
+}
+
+__device__ inline void load_weights(FwdWeights& s, const unsigned short* wbf, const float* theta) {
+    swizzle_layer<kEnc>(s.w1, wbf + kW1, kHid / 16);
+    swizzle_layer<kHid>(s.w2, wbf + kW2, kHid / 16);
+    swizzle_layer<kHid>(s.w3, wbf + kW3, kHid / 16);
+    swizzle_layer<kHid>(s.w4, wbf + kW4, kOutP / 16);
+    for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+        s.b1[i] = theta[kB1 + i];
+        s.b2[i] = theta[kB2 + i];
+        s.b3[i] = theta[kB3 + i];
+    }
+    for (int i = threadIdx.x; i < kOutP; i += blockDim.x) s.b4[i] = theta[kB4 + i];
+}
+
+// D += A B with A the lane's 8 weights of block blk, B two k-tiles of data.
+__device__ inline f32x4 mfma_w(const uint4* blk, bf16x4 b0, bf16x4 b1, f32x4 c) {
+    const uint4 v = blk[__lane_id()];
+    const bf16x8s b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, v), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
 // x[kt]: features 16 kt + 4 g + j of sample r (bf16); y[t]: raw outputs
 // 16 t + 4 g + i of sample r (f32), t = 0..2 (outputs 40..47 are padding).
-__device__ inline void mlp_cols_fwd(const SharedWeights& s, const bf16x4 x[2], f32x4 y[3]) {
-    const int l = __lane_id(), r = l & 15, g = l >> 4;
+__device__ inline void mlp_cols_fwd(const FwdWeights& s, const bf16x4 x[2], f32x4 y[3]) {
+    const int g = __lane_id() >> 4;
     bf16x4 ha[4], hb[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        f32x4 acc = bias4(s.b1 + 16 * t + 4 * g);
-#pragma unroll
-        for (int k = 0; k < 2; k += 2)
-            acc = mfma_k32(ld4(s.w1 + (16 * t + r) * kEnc + 16 * k + 4 * g),
-                           ld4(s.w1 + (16 * t + r) * kEnc + 16 * (k + 1) + 4 * g), x[k], x[k + 1], acc);
-        ha[t] = pack_relu(acc);
-    }
+    for (int t = 0; t < 4; ++t) ha[t] = pack_relu(mfma_w(s.w1 + 64 * t, x[0], x[1], bias4(s.b1 + 16 * t + 4 * g)));
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         f32x4 acc = bias4(s.b2 + 16 * t + 4 * g);
 #pragma unroll
-        for (int k = 0; k < 4; k += 2)
-            acc = mfma_k32(ld4(s.w2 + (16 * t + r) * kHid + 16 * k + 4 * g),
-                           ld4(s.w2 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), ha[k], ha[k + 1], acc);
+        for (int p = 0; p < 2; ++p) acc = mfma_w(s.w2 + 64 * (2 * t + p), ha[2 * p], ha[2 * p + 1], acc);
         hb[t] = pack_relu(acc);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         f32x4 acc = bias4(s.b3 + 16 * t + 4 * g);
 #pragma unroll
-        for (int k = 0; k < 4; k += 2)
-            acc = mfma_k32(ld4(s.w3 + (16 * t + r) * kHid + 16 * k + 4 * g),
-                           ld4(s.w3 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), hb[k], hb[k + 1], acc);
+        for (int p = 0; p < 2; ++p) acc = mfma_w(s.w3 + 64 * (2 * t + p), hb[2 * p], hb[2 * p + 1], acc);
         ha[t] = pack_relu(acc);
     }
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
         f32x4 acc = bias4(s.b4 + 16 * t + 4 * g);
 #pragma unroll
-        for (int k = 0; k < 4; k += 2)
-            acc = mfma_k32(ld4(s.w4 + (16 * t + r) * kHid + 16 * k + 4 * g),
-                           ld4(s.w4 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), ha[k], ha[k + 1], acc);
+        for (int p = 0; p < 2; ++p) acc = mfma_w(s.w4 + 64 * (2 * t + p), ha[2 * p], ha[2 * p + 1], acc);
         y[t] = acc;
     }
 }
@@ -246,7 +279,7 @@ struct HalfRow {
 // samples 16 q .. 16 q + 15) calls out(q, y) with lane (r, g) holding outputs
 // 16 t + 4 g + i of sample 16 q + r.
 template <typename Out>
-__device__ inline void mlp_stage_fwd(const SharedWeights& s, const WaveStage& st, Out out) {
+__device__ inline void mlp_stage_fwd(const FwdWeights& s, const WaveStage& st, Out out) {
     const int l = __lane_id(), r = l & 15, g = l >> 4;
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
@@ -257,6 +290,7 @@ __device__ inline void mlp_stage_fwd(const SharedWeights& s, const WaveStage& st
         out(q, y);
     }
 }
+
 
 // Mixture parameters of one raw output row.
 struct Lobe {

@@ -20,8 +20,8 @@ __global__ __launch_bounds__(kBlock) void k_guide_eval(const float* __restrict__
                                                        const float* __restrict__ q, uint32_t stride, uint32_t n,
                                                        float* __restrict__ out, GuideBox box) {
     extern __shared__ float dyn[];
-    SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
-    WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
+    FwdWeights& sw = *reinterpret_cast<FwdWeights*>(dyn);
+    WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(FwdWeights));
     load_weights(sw, wbf, theta);
     __syncthreads();
     const int wave = threadIdx.x >> 6, r = __lane_id() & 15;
@@ -116,8 +116,8 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
                                                       uint32_t n16, GuideCtl* __restrict__ ctl, GuideBox box,
                                                       TrainScratch ts) {
     extern __shared__ float dyn[];
-    SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
-    WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
+    FwdWeights& sw = *reinterpret_cast<FwdWeights*>(dyn);
+    WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(FwdWeights));
     float* ystage = reinterpret_cast<float*>(stages + kBlock / 64);  // [4 waves][64][kOut]
     load_weights(sw, wbf, theta);
     const int wave = threadIdx.x >> 6, l = __lane_id(), r = l & 15, g = l >> 4;
@@ -145,21 +145,14 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
         bf16x4 ha[4], hb[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            f32x4 acc = bias4(sw.b1 + 16 * t + 4 * g);
-#pragma unroll
-            for (int k = 0; k < 2; k += 2)
-                acc = mfma_k32(ld4(sw.w1 + (16 * t + r) * kEnc + 16 * k + 4 * g),
-                               ld4(sw.w1 + (16 * t + r) * kEnc + 16 * (k + 1) + 4 * g), x[k], x[k + 1], acc);
-            ha[t] = pack_relu(acc);
+            ha[t] = pack_relu(mfma_w(sw.w1 + 64 * t, x[0], x[1], bias4(sw.b1 + 16 * t + 4 * g)));
             if (keep) st_col4(ts.h1T, ts.rows, 16 * t + 4 * g, sq, ha[t]);
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             f32x4 acc = bias4(sw.b2 + 16 * t + 4 * g);
 #pragma unroll
-            for (int k = 0; k < 4; k += 2)
-                acc = mfma_k32(ld4(sw.w2 + (16 * t + r) * kHid + 16 * k + 4 * g),
-                               ld4(sw.w2 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), ha[k], ha[k + 1], acc);
+            for (int p = 0; p < 2; ++p) acc = mfma_w(sw.w2 + 64 * (2 * t + p), ha[2 * p], ha[2 * p + 1], acc);
             hb[t] = pack_relu(acc);
             if (keep) st_col4(ts.h2T, ts.rows, 16 * t + 4 * g, sq, hb[t]);
         }
@@ -167,9 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
         for (int t = 0; t < 4; ++t) {
             f32x4 acc = bias4(sw.b3 + 16 * t + 4 * g);
 #pragma unroll
-            for (int k = 0; k < 4; k += 2)
-                acc = mfma_k32(ld4(sw.w3 + (16 * t + r) * kHid + 16 * k + 4 * g),
-                               ld4(sw.w3 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), hb[k], hb[k + 1], acc);
+            for (int p = 0; p < 2; ++p) acc = mfma_w(sw.w3 + 64 * (2 * t + p), hb[2 * p], hb[2 * p + 1], acc);
             ha[t] = pack_relu(acc);
             if (keep) st_col4(ts.h3T, ts.rows, 16 * t + 4 * g, sq, ha[t]);
         }
@@ -177,9 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_train_fwd(const float* __restrict__ 
         for (int t = 0; t < 3; ++t) {
             f32x4 acc = bias4(sw.b4 + 16 * t + 4 * g);
 #pragma unroll
-            for (int k = 0; k < 4; k += 2)
-                acc = mfma_k32(ld4(sw.w4 + (16 * t + r) * kHid + 16 * k + 4 * g),
-                               ld4(sw.w4 + (16 * t + r) * kHid + 16 * (k + 1) + 4 * g), ha[k], ha[k + 1], acc);
+            for (int p = 0; p < 2; ++p) acc = mfma_w(sw.w4 + 64 * (2 * t + p), ha[2 * p], ha[2 * p + 1], acc);
             if (16 * t + 4 * g < kOut)
                 for (int i = 0; i < 4; ++i) ys[(16 * q + r) * kOut + 16 * t + 4 * g + i] = acc[i];
         }
@@ -394,9 +383,9 @@ __global__ void k_guide_tobf16(const float* __restrict__ theta, unsigned short* 
     if (i < (uint32_t)kParams) wbf[i] = f2bf(theta[i]);
 }
 
-size_t eval_lds_bytes() { return sizeof(SharedWeights) + sizeof(WaveStage) * (kBlock / 64); }
+size_t eval_lds_bytes() { return sizeof(FwdWeights) + sizeof(WaveStage) * (kBlock / 64); }
 size_t fwd_lds_bytes() {
-    return sizeof(SharedWeights) + sizeof(WaveStage) * (kBlock / 64) + sizeof(float) * 64 * kOut * (kBlock / 64);
+    return sizeof(FwdWeights) + sizeof(WaveStage) * (kBlock / 64) + sizeof(float) * 64 * kOut * (kBlock / 64);
 }
 
 // The inference and forward kernels use more than the default 64 KiB of

@@ -438,6 +438,7 @@ __global__ void k_count_guided(RenderArgs a, Work w, uint32_t nb) {
     if (q) atomicAdd(reinterpret_cast<unsigned long long*>(w.cnt + 30), q);
 }
 
+// Launch shape of the guide inference kernel.
 // Guide inference for the live hits of buffer A whose material is guidable,
 // in shading order: each wave encodes 64 hits (one per lane) and runs the
 // register-resident MLP; raw outputs go to gout[slot] as half floats.
@@ -447,8 +448,8 @@ __global__ __launch_bounds__(kBlock) void k_guide_paths(RenderArgs a, PathBuf A,
                                                         const unsigned short* __restrict__ wbf, GuideBox box) {
     using namespace guide;
     extern __shared__ float dyn[];
-    SharedWeights& sw = *reinterpret_cast<SharedWeights*>(dyn);
-    WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(SharedWeights));
+    FwdWeights& sw = *reinterpret_cast<FwdWeights*>(dyn);
+    WaveStage* stages = reinterpret_cast<WaveStage*>(reinterpret_cast<char*>(dyn) + sizeof(FwdWeights));
     const uint32_t n = *cntA;
     if (n == 0) return;
     load_weights(sw, wbf, theta);
@@ -804,7 +805,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 size_t guide_paths_lds_bytes() {
-    return sizeof(guide::SharedWeights) + sizeof(guide::WaveStage) * (kBlock / 64);
+    return sizeof(guide::FwdWeights) + sizeof(guide::WaveStage) * (kBlock / 64);
 }
 
 }  // namespace
