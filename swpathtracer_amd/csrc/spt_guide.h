@@ -131,15 +131,26 @@ __device__ inline bf16x4 ld4s(const unsigned short* p, int stride) {
 // of sample r in each output tile t, which is exactly the B-operand layout of
 // the next layer's k-tile t: activations never leave registers.
 // ---------------------------------------------------------------------------
-__device__ inline unsigned short f2bf_relu(float v) { return f2bf(fmaxf(v, 0.0f)); }
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Two floats as packed bfloat16 (a in the low half) by gfx950's
+// v_cvt_pk_bf16_f32, round to nearest even: the same bits as f2bf for every
+// non-NaN float.
+__device__ inline uint32_t pack_bf2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+// ReLU of two packed bfloat16: a negative value (sign bit set) is a negative
+// int16, so an int16 max with 0 gives +0 (the bits of f2bf(fmaxf(x, 0)) for
+// every non-NaN x).
+__device__ inline uint32_t relu_bf2(uint32_t v) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2, v), (i16x2){0, 0}));
+}
 
 __device__ inline bf16x4 pack_relu(f32x4 v) {
-    bf16x4 r;
-    r[0] = (short)f2bf_relu(v[0]);
-    r[1] = (short)f2bf_relu(v[1]);
-    r[2] = (short)f2bf_relu(v[2]);
-    r[3] = (short)f2bf_relu(v[3]);
-    return r;
+    return __builtin_bit_cast(bf16x4, (u32x2){relu_bf2(pack_bf2(v[0], v[1])), relu_bf2(pack_bf2(v[2], v[3]))});
 }
 
 __device__ inline f32x4 bias4(const float* b) { return f32x4{b[0], b[1], b[2], b[3]}; }
@@ -244,8 +255,7 @@ __device__ inline void stage_row(WaveStage& st, const float f[kEnc]) {
     uint2* row = reinterpret_cast<uint2*>(st.x + __lane_id() * kStageRow);
 #pragma unroll
     for (int i = 0; i < kEnc / 4; ++i)
-        row[i] = make_uint2((uint32_t)f2bf(f[4 * i]) | ((uint32_t)f2bf(f[4 * i + 1]) << 16),
-                            (uint32_t)f2bf(f[4 * i + 2]) | ((uint32_t)f2bf(f[4 * i + 3]) << 16));
+        row[i] = make_uint2(pack_bf2(f[4 * i], f[4 * i + 1]), pack_bf2(f[4 * i + 2], f[4 * i + 3]));
 }
 
 // Writes the 40 raw outputs of one sample from the column-form result.
