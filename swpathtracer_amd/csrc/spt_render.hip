@@ -767,14 +767,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const V3 wgt = (bsdf * wi.y) / pdf;
         uint4 gr = (GUIDED && a.train) ? A.grec[i] : make_uint4(0u, 0u, 0u, 0u);
         const bool cand = GUIDED && a.train && guided && via && gr.w < 3u && wi.y > 0.0f && bsdf.x + bsdf.y + bsdf.z > 0.0f;
-        if (GUIDED && a.train) {  // candidates counted per wave (one atomic)
-            const uint64_t cb = __ballot(cand);
-            if (cb && __lane_id() == (unsigned)(__ffsll((unsigned long long)cb) - 1))
-                atomicAdd(a.rec_count + 1, (uint32_t)__popcll(cb));
+        const bool take = cand && (a.rec_stride <= 1u ||
+                                   mix64(key + (uint64_t)(depth + 1) * 0x9E3779B97F4A7C15ull) % a.rec_stride == 0u);
+        uint32_t rs = 0;
+        if (GUIDED && a.train) {
+            // one 64-bit atomic per wave adds the wave's candidates (high
+            // word) and records (low word); the record slots of the wave are
+            // consecutive from the old low word
+            const uint64_t cb = __ballot(cand), tb = __ballot(take);
+            const int leader = __ffsll((unsigned long long)cb) - 1;
+            const unsigned long long inc = ((unsigned long long)__popcll(cb) << 32) | (unsigned long long)__popcll(tb);
+            unsigned long long* rc = reinterpret_cast<unsigned long long*>(a.rec_count);
+            if (tb) {
+                unsigned long long old = 0;
+                if (__lane_id() == (unsigned)leader) old = atomicAdd(rc, inc);
+                rs = __shfl((uint32_t)old, leader) + (uint32_t)__popcll(tb & __lanemask_lt());
+            } else if (cb && __lane_id() == (unsigned)leader) {
+                atomicAdd(rc, inc);
+            }
         }
-        if (cand && (a.rec_stride <= 1u ||
-                     mix64(key + (uint64_t)(depth + 1) * 0x9E3779B97F4A7C15ull) % a.rec_stride == 0u)) {
-            uint32_t rs = atomicAdd(a.rec_count, 1u);
+        if (take) {
             if (a.rec_ring) rs %= a.rec_cap;
             if (rs < a.rec_cap) {
                 float* rec = a.records + (size_t)rs * guide::kRecord;
