@@ -316,8 +316,7 @@ def guide_experiment(args):
     # (32x64 + 64x64 + 64x64 + 64x40 multiply-adds)
     # (one wavefront, so that no other wavefront's kernels share the timed
     # launches)
-    tn0 = dev.get_tuning()
-    dev.set_tuning(**{**tn0, "wavefronts": 1})
+    dev.set_tuning(wavefronts=1, path_capacity=args.guide_capacity)
     p = integ.params()
     p.spp, p.seed = 144, 0x1FE12
     p.flags = L.SPT_RENDER_FILM32 | L.SPT_RENDER_GUIDED | L.SPT_RENDER_TIME_KERNELS
@@ -325,7 +324,7 @@ def guide_experiment(args):
     dev.render(cam, p, film, None, None)
     dev.sync()
     rs = dev.render_stats()
-    dev.set_tuning(**tn0)
+    dev.set_tuning(path_capacity=args.guide_capacity)
     inf_tflops = rs["guide_queries"] * 25600 / max(rs["guide_ms"] * 1e-3, 1e-12) / 1e12
     infer = {"queries": int(rs["guide_queries"]), "kernel_ms": round(rs["guide_ms"], 3),
              "tflops": round(inf_tflops, 1), "mfma_util": round(inf_tflops / 2500.0, 4)}
@@ -524,7 +523,7 @@ def main():
     # wavefront (the kernel with the GPU to itself).
     aggregate = bytes_per_ray * rays / wall / 1e9
     tn_default = dev.get_tuning()
-    dev.set_tuning(**{**tn_default, "wavefronts": 1})
+    dev.set_tuning(wavefronts=1)  # the other fields at their one-wavefront defaults
     step(0, L.SPT_RENDER_FILM32 | L.SPT_RENDER_TIME_KERNELS)
     dev.sync()
     si = dev.render_stats()
