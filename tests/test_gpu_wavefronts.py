@@ -62,3 +62,16 @@ def test_training_records_are_shared_across_wavefronts(device, box):
         rec = device.guide_records()
         assert np.all(np.isfinite(rec[:, :13]))
     assert counts[1] == counts[3] and counts[1][0] > 1000
+
+
+def test_launch_shapes_follow_the_wavefront_count(device):
+    # zero fields resolve by wavefront count; fields the caller sets are kept
+    device.set_tuning()
+    t = device.get_tuning()
+    assert (t["wavefronts"], t["shade_blocks_per_cu"], t["refill_lanes"]) == (3, 64, 20)
+    device.set_tuning(wavefronts=1)
+    t = device.get_tuning()
+    assert (t["wavefronts"], t["shade_blocks_per_cu"], t["refill_lanes"]) == (1, 128, 48)
+    device.set_tuning(wavefronts=2, refill_lanes=30)
+    t = device.get_tuning()
+    assert (t["wavefronts"], t["shade_blocks_per_cu"], t["refill_lanes"]) == (2, 64, 30)
