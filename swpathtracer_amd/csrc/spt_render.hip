@@ -71,7 +71,7 @@ struct RenderArgs {
     float bsdf_frac;
     unsigned short* gout;     // per path slot: 40 raw network outputs (IEEE half)
     float* records;           // training records (guide::kRecord floats each)
-    uint32_t* rec_count;  // [0]: records written, [1]: candidate vertices
+    uint32_t* rec_count;  // [0]: records written, [1]: candidate vertices (one 8-B aligned u64)
     uint32_t* rec_shared; // concurrent wavefronts: one rec_count for all (else nullptr: the work words)
     const float* theta;          // guide parameters (FP32 master copy)
     const unsigned short* wbf;   // guide weights in bf16
@@ -438,7 +438,6 @@ __global__ void k_count_guided(RenderArgs a, Work w, uint32_t nb) {
     if (q) atomicAdd(reinterpret_cast<unsigned long long*>(w.cnt + 30), q);
 }
 
-// Launch shape of the guide inference kernel.
 // Guide inference for the live hits of buffer A whose material is guidable,
 // in shading order: each wave encodes 64 hits (one per lane) and runs the
 // register-resident MLP; raw outputs go to gout[slot] as half floats.
