@@ -354,26 +354,40 @@ __global__ __launch_bounds__(kBlock) void k_hist(RenderArgs a, Work w, const uin
     for (uint32_t k = threadIdx.x; k < a.keys; k += kBlock) w.bhist[k * gridDim.x + blockIdx.x] = hist[k];
 }
 
+// Exclusive prefix sum of the [key][chunk] counts in place, one block:
+// tiles of kScanThreads counts read and written coalesced, each scanned by
+// wave shuffles and one pass over the wave totals, with a running carry.
 __global__ __launch_bounds__(kScanThreads) void k_scan(RenderArgs a, Work w, uint32_t nb) {
-    __shared__ uint32_t part[kScanThreads];
+    constexpr uint32_t kWaves = kScanThreads / 64;
+    __shared__ uint32_t wbase[kWaves];
+    __shared__ uint32_t tile_sum;
     const uint32_t total = a.keys * nb;
-    const uint32_t per = (total + kScanThreads - 1) / kScanThreads;
-    const uint32_t b0 = min(total, threadIdx.x * per), b1 = min(total, b0 + per);
-    uint32_t sum = 0;
-    for (uint32_t i = b0; i < b1; ++i) sum += w.bhist[i];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {  // inclusive Hillis-Steele scan
-        const uint32_t v = (threadIdx.x >= off) ? part[threadIdx.x - off] : 0u;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (uint32_t t0 = 0; t0 < total; t0 += kScanThreads) {
+        const uint32_t i = t0 + threadIdx.x;
+        const uint32_t v = i < total ? w.bhist[i] : 0u;
+        uint32_t x = v;  // inclusive scan within the wave
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wbase[wave] = x;
         __syncthreads();
-        part[threadIdx.x] += v;
+        if (wave == 0) {  // exclusive scan of the wave totals
+            const uint32_t s = lane < kWaves ? wbase[lane] : 0u;
+            uint32_t t = s;
+            for (uint32_t off = 1; off < kWaves; off <<= 1) {
+                const uint32_t y = __shfl_up(t, off);
+                if (lane >= off) t += y;
+            }
+            if (lane < kWaves) wbase[lane] = t - s;
+            if (lane == kWaves - 1) tile_sum = t;
+        }
         __syncthreads();
-    }
-    uint32_t acc = part[threadIdx.x] - sum;
-    for (uint32_t i = b0; i < b1; ++i) {
-        const uint32_t c = w.bhist[i];
-        w.bhist[i] = acc;
-        acc += c;
+        if (i < total) w.bhist[i] = carry + wbase[wave] + x - v;
+        carry += tile_sum;
+        __syncthreads();  // wbase and tile_sum are rewritten by the next tile
     }
 }
 
