@@ -347,8 +347,9 @@ def guide_experiment(args):
 def batch_bench(args):
     """64 independent scenes (box + seeded icosphere or displaced sphere,
     1k-500k triangles log-uniform, 128^2-1024^2, 16/64/256 spp, Lambertian /
-    GGX / dielectric objects) rendered by one wavefront after autotuning tile
-    and queue sizes on a 1/64 sample of every scene's samples."""
+    GGX / dielectric objects) rendered in one call after autotuning tile and
+    queue sizes, launch shapes and the number of concurrent wavefronts on a
+    1/64 sample of every scene's samples."""
     from swpathtracer_amd import api, scenes
 
     t0 = time.perf_counter()

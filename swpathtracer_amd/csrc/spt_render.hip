@@ -1011,7 +1011,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
     const uint32_t refill_grid = std::min<uint32_t>(blocks_for(cap), (uint32_t)ctx->num_cus * 8);
     // sort chunks: many (latency hiding) unless many keys would make the scan long
     const uint32_t sort_blocks =
-        a.keys ? std::max<uint32_t>(256, std::min<uint32_t>(kMaxSortBlocks, 16384 / a.keys)) : 256;
+        a.keys ? std::max<uint32_t>(1024, std::min<uint32_t>(kMaxSortBlocks, 16384 / a.keys)) : 256;
     spt_status s = ensure_spill(ctx, (size_t)ext_grid * kBlock *
                                          (a.sc.stack_cap4 > kLdsStack ? a.sc.stack_cap4 - kLdsStack : 1));
     if (s != SPT_OK) return s;
