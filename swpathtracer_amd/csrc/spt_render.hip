@@ -1075,7 +1075,7 @@ spt_status run_paths(spt_ctx* ctx, RenderArgs& a, uint64_t total, const spt_tuni
                     (void)hipEventRecord(e0, st);
                 }
             }
-            hipLaunchKernelGGL(k_guide_paths, dim3((uint32_t)ctx->num_cus * 2), dim3(kBlock), guide_paths_lds_bytes(),
+            hipLaunchKernelGGL(k_guide_paths, dim3((uint32_t)ctx->num_cus * 3), dim3(kBlock), guide_paths_lds_bytes(),
                                st, a, A, w, cntA, a.theta, a.wbf, a.gbox);
             if (timed && !gev.empty()) (void)hipEventRecord(gev.back(), st);
         }
