@@ -326,8 +326,13 @@ def guide_experiment(args):
     rs = dev.render_stats()
     dev.set_tuning(path_capacity=args.guide_capacity)
     inf_tflops = rs["guide_queries"] * 25600 / max(rs["guide_ms"] * 1e-3, 1e-12) / 1e12
+    # its memory side: per query 104 B read (queue index 4, hit 16, sort key 4,
+    # origin 16, direction 16, the triangle's shading record 48) and 80 B of
+    # half-float outputs written
+    inf_gbs = rs["guide_queries"] * 184 / max(rs["guide_ms"] * 1e-3, 1e-12) / 1e9
     infer = {"queries": int(rs["guide_queries"]), "kernel_ms": round(rs["guide_ms"], 3),
-             "tflops": round(inf_tflops, 1), "mfma_util": round(inf_tflops / 2500.0, 4)}
+             "tflops": round(inf_tflops, 1), "mfma_util": round(inf_tflops / 2500.0, 4),
+             "algorithmic_gbs": round(inf_gbs, 1), "hbm_frac": round(inf_gbs / HBM_PEAK_GBS, 4)}
     res = {"scene": f"slit box, {sc.n_tris} triangles, {W}x{H}", "reference_spp": osp,
            "plain_ms": round(t_plain * 1e3, 2), "plain_rmse": round(r_plain, 6),
            "plain_mpaths_s": round(npix * 1024 / t_plain / 1e6, 3),
